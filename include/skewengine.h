@@ -1,0 +1,139 @@
+/*
+ * skewengine.h -- C ABI of the MI355X-native skewEngine (libskewengine.so).
+ *
+ * Plain C: pointers, sizes and int status codes only.  Every entry point
+ * names the reference interface it replaces (paths relative to the
+ * reference repository root).  All grids are row-major float32, dimx
+ * columns by dimy rows, values in the reference's units (DEM heights in
+ * cell units, i.e. metres / cell size).
+ *
+ * Status: 0 on success, a negative SKE_E* code otherwise; the message of the
+ * last failure on the calling thread is available from ske_last_error().
+ * Engine handles are not thread-safe; distinct handles may be used from
+ * distinct threads (each owns one HIP stream).
+ */
+#ifndef SKEWENGINE_H
+#define SKEWENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKE_OK 0
+#define SKE_EINVAL (-1)      /* bad argument (null pointer, size, sector, operator id) */
+#define SKE_EHIP (-2)        /* HIP runtime error */
+#define SKE_ENOMEM (-3)      /* device or host allocation failed */
+#define SKE_ESTATE (-4)      /* call out of order (e.g. kernel() before skew()) */
+#define SKE_EUNSUPPORTED (-5) /* valid request this build does not support */
+
+/* Operator ids: enum KERNELS (src/global.h:77). */
+#define SKE_KERNEL_SDEM 0        /* total viewshed, kernelV3 (src/kernelSDEM.h:168) */
+#define SKE_KERNEL_IDENTITYDEM 1 /* identity on a DEM (src/kernelUNIT.h:16) */
+#define SKE_KERNEL_RADON 2       /* global Radon / FBP (src/kernelRADN.h:213) */
+#define SKE_KERNEL_IDENTITYIMG 3 /* identity on an image (src/kernelUNIT.h:16) */
+#define SKE_KERNEL_CEPSTRUM 4    /* cepstral blur (src/kernelBLUR.h:267) */
+
+#define SKE_MAX_SECTOR 180 /* maxSector (src/global.cpp:69) */
+
+/* Per-sector public fields after skew(angle) (src/skewEngine.cuh:356-367). */
+typedef struct ske_sector_params {
+    int a, newAngle, sectorType;
+    int dim_i, dim_o, offset, skewHeight, skewWidth;
+    double scale, skewness;
+} ske_sector_params;
+
+/* Reference device-operator form (the kernelcuda pointer,
+ * src/skewEngine.cuh:369): a host function that launches the user's HIP
+ * kernel on `stream` (a hipStream_t) with device pointers.  val is the
+ * observer height the reference passes as POVh, angle is newAngle. */
+typedef void (*ske_device_operator)(float *d_skewOutput, float *d_skewInput, int dim_i, int skewHeight,
+                                    unsigned short *d_last, unsigned short *d_first, float val, int angle,
+                                    void *stream, void *user);
+
+typedef struct ske_engine ske_engine;
+
+const char *ske_last_error(void);
+int ske_version(void);
+int ske_device_count(int *count);
+
+/* DLL entry point: extern "C" int run(int argc, char *argv[])
+ * (src/main.cu:199-202).  argv: [--kernel ID] [--input FILE(.flt|.bil|.png)]
+ * [--dims WxH] [--output FILE.flt] [--sectors A:B] [--povh H] [--device D].
+ * With no arguments it runs the total viewshed on a built-in 500x500
+ * synthetic DEM.  Returns 0 on success. */
+int run(int argc, char **argv);
+
+/* Whole-image path: allocate + prepare + the sector farm + reduce
+ * (src/main.cu:71-172,186; src/skewEngine.cuh:388-411,861-914) for sectors
+ * [sector_begin, sector_end) on device 0.  input/output are host buffers of
+ * dimx*dimy floats; output is overwritten. */
+int ske_execute(int kernel_id, int dimx, int dimy, const float *input, float *output, float povh,
+                int sector_begin, int sector_end);
+
+/* skewEngine<float>(dimx, dimy, inData, isGPU=true, deviceId)
+ * (src/skewEngine.cuh:281,461) together with allocate/prepare (:388-411,
+ * :962-987) and skewAlloc (:418-456).  `input` is a host grid; the engine
+ * keeps the four oriented copies in device memory. */
+int ske_create(int dimx, int dimy, const float *input, int device_id, ske_engine **out);
+/* Same, with the grid already resident in device memory (copied). */
+int ske_create_device(int dimx, int dimy, const float *d_input, int device_id, ske_engine **out);
+/* ~skewEngine (src/skewEngine.cuh:484-501) + deallocate (:378-386). */
+int ske_destroy(ske_engine *e);
+
+/* Operator selection: the kernelcpu/kernelcuda assignment of
+ * src/main.cu:107-136.  povh is POVh (observer height in cell units,
+ * src/kernelSDEM.h:17,159), used by SKE_KERNEL_SDEM. */
+int ske_set_kernel(ske_engine *e, int kernel_id, float povh);
+/* Replace the operator by a user device operator (kernelcuda form). */
+int ske_set_device_operator(ske_engine *e, ske_device_operator op, void *user);
+
+/* skew(angle) (src/skewEngine.cuh:605-682): computeParams, skewShape,
+ * resetSkew and the skewed copy of the oriented input. */
+int ske_skew(ske_engine *e, int angle);
+/* kernel() (src/skewEngine.cuh:694-746): run the operator on the skewed rows. */
+int ske_kernel(ske_engine *e);
+/* deskew(isMax) (src/skewEngine.cuh:755-793) into the engine's orientation
+ * accumulator of the current sector type. */
+int ske_deskew(ske_engine *e, int is_max);
+/* reduce(output, isMax) (src/skewEngine.cuh:861-914): output (host, dimx*dimy)
+ * is accumulated into (+ or max) from the orientation accumulators. */
+int ske_reduce(ske_engine *e, float *output, int is_max);
+
+/* Inspection of the current sector (public fields of skewEngine<T>). */
+int ske_get_params(ske_engine *e, ske_sector_params *p);
+/* target/weight: dim_i entries; first/last: skewHeight entries. */
+int ske_get_tables(ske_engine *e, int *target, float *weight, unsigned short *first, unsigned short *last);
+/* skewInput / skewOutput rows (skewHeight*dim_i floats each; either may be NULL). */
+int ske_get_skew(ske_engine *e, float *skewInput, float *skewOutput);
+/* Device pointers of the current sector's buffers (valid until the next skew()). */
+int ske_device_pointers(ske_engine *e, float **d_skewInput, float **d_skewOutput, unsigned short **d_first,
+                        unsigned short **d_last);
+
+/* All sectors [sector_begin, sector_end) of the engine's grid through
+ * skew -> operator -> deskew -> reduce in one call (the OpenMP farm of
+ * src/main.cu:145-164 with a deterministic ascending sector order).  The
+ * deskew mode follows the operator (max for SKE_KERNEL_CEPSTRUM, as
+ * main.cu:156).  The result stays on the device (ske_output_device) and is
+ * copied to `output` (host, dimx*dimy floats) when it is not NULL. */
+int ske_run(ske_engine *e, int kernel_id, int sector_begin, int sector_end, float povh, float *output);
+int ske_output_device(ske_engine *e, const float **d_output);
+int ske_synchronize(ske_engine *e);
+
+/* Options (no reference counterpart). */
+#define SKE_OPT_SDEM_VARIANT 1  /* 0: fma-based exact division (default), 1: IEEE divide */
+#define SKE_OPT_ARENA_BYTES 2   /* device budget for one chunk of skewOutput rows */
+#define SKE_OPT_PROFILE 3       /* 1: time every launch with HIP events on the engine stream */
+int ske_set_option(ske_engine *e, int option, long long value);
+/* Accumulated HIP-event time of the launches named `name` since the last
+ * reset (profiling option on): total milliseconds and launch count. */
+int ske_profile_get(ske_engine *e, const char *name, double *total_ms, int *count);
+int ske_profile_reset(ske_engine *e);
+
+/* Work counts of the exact maps for sectors [s0,s1) of a dimx x dimy grid:
+ * cells = sum of row lengths, steps = sum L(L-1) (viewshed ray steps). */
+int ske_work(int dimx, int dimy, int s0, int s1, double *cells, double *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SKEWENGINE_H */

@@ -1,0 +1,93 @@
+"""ctypes binding of libskewengine.so (the C ABI of include/skewengine.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libskewengine.so")
+
+SKE_OK = 0
+KERNEL_SDEM, KERNEL_IDENTITYDEM, KERNEL_RADON, KERNEL_IDENTITYIMG, KERNEL_CEPSTRUM = 0, 1, 2, 3, 4
+OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE = 1, 2, 3
+MAX_SECTOR = 180
+
+# Every symbol include/skewengine.h declares.
+EXPORTS = [
+    "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_create",
+    "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
+    "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_get_skew",
+    "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option",
+    "ske_profile_get", "ske_profile_reset", "ske_work",
+]
+
+
+class SectorParams(ctypes.Structure):
+    """ske_sector_params: the public per-sector fields of skewEngine<T>
+    (src/skewEngine.cuh:356-367)."""
+    _fields_ = [
+        ("a", ctypes.c_int), ("newAngle", ctypes.c_int), ("sectorType", ctypes.c_int),
+        ("dim_i", ctypes.c_int), ("dim_o", ctypes.c_int), ("offset", ctypes.c_int),
+        ("skewHeight", ctypes.c_int), ("skewWidth", ctypes.c_int),
+        ("scale", ctypes.c_double), ("skewness", ctypes.c_double),
+    ]
+
+
+class SkeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("skewengine error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libskewengine.so (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libskewengine.so is not built (run `python -m skewengine_amd.build`): " + LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    f32p, f64p, i32p, u16p = P(ctypes.c_float), P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_uint16)
+    L.ske_last_error.restype = ctypes.c_char_p
+    L.ske_last_error.argtypes = []
+    L.ske_version.restype = ctypes.c_int
+    L.ske_device_count.argtypes = [i32p]
+    L.run.argtypes = [ctypes.c_int, P(ctypes.c_char_p)]
+    L.ske_execute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_int,
+                              ctypes.c_int]
+    L.ske_create.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, P(vp)]
+    L.ske_create_device.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, P(vp)]
+    L.ske_destroy.argtypes = [vp]
+    L.ske_set_kernel.argtypes = [vp, ctypes.c_int, ctypes.c_float]
+    L.ske_set_device_operator.argtypes = [vp, vp, vp]
+    L.ske_skew.argtypes = [vp, ctypes.c_int]
+    L.ske_kernel.argtypes = [vp]
+    L.ske_deskew.argtypes = [vp, ctypes.c_int]
+    L.ske_reduce.argtypes = [vp, vp, ctypes.c_int]
+    L.ske_get_params.argtypes = [vp, P(SectorParams)]
+    L.ske_get_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.ske_get_skew.argtypes = [vp, vp, vp]
+    L.ske_device_pointers.argtypes = [vp, P(vp), P(vp), P(vp), P(vp)]
+    L.ske_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp]
+    L.ske_output_device.argtypes = [vp, P(vp)]
+    L.ske_synchronize.argtypes = [vp]
+    L.ske_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.ske_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
+    L.ske_profile_reset.argtypes = [vp]
+    L.ske_work.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p, f64p]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != SKE_OK:
+        msg = lib().ske_last_error()
+        raise SkeError(rc, msg.decode() if msg else "")
+    return rc

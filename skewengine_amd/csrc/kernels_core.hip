@@ -1,0 +1,252 @@
+// kernels_core.hip -- orientation copies, skew, identity operator, deskew and
+// reduce on gfx950 (wave64).  Float arithmetic follows the reference CPU
+// path operation by operation (built with -ffp-contract=off), so the
+// per-sector and accumulated results are bit-identical to a one-thread run
+// of the reference.
+#include "device_common.h"
+#include "launch.h"
+
+namespace ske {
+
+// ---------------------------------------------------------------------------
+// prepare (src/skewEngine.cuh:962-987).  32x32 tiles through LDS so that the
+// two transposed copies are written coalesced.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ in0, float* __restrict__ in1,
+                                                 float* __restrict__ in2, float* __restrict__ in3,
+                                                 int dimx, int dimy) {
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int i = y0 + k, j = x0 + tx;
+        if (i < dimy && j < dimx) {
+            const float v = in0[(size_t)i * dimx + j];
+            tile[k][tx] = v;
+            in3[(size_t)i * dimx + (dimx - 1 - j)] = v;  // in3[dimx*i + cj]
+        }
+    }
+    __syncthreads();
+    // in1[dimy*cj + ci] and in2[dimy*j + ci]: output row indexed by x, column by y
+    for (int k = ty; k < 32; k += 8) {
+        const int j = x0 + k;       // source column -> output row
+        const int i = y0 + tx;      // source row -> output column
+        if (i < dimy && j < dimx) {
+            const float v = tile[tx][k];
+            const int ci = dimy - 1 - i, cj = dimx - 1 - j;
+            in1[(size_t)dimy * cj + ci] = v;
+            in2[(size_t)dimy * j + ci] = v;
+        }
+    }
+}
+
+hipError_t launch_prepare(const float* in0, float* in1, float* in2, float* in3, int dimx, int dimy,
+                          hipStream_t s) {
+    dim3 grid((dimx + 31) / 32, (dimy + 31) / 32);
+    hipLaunchKernelGGL(k_prepare, grid, dim3(256), 0, s, in0, in1, in2, in3, dimx, dimy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// skew, materialised: every cell of the H x dim_i skewed block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_skew_full(const SecDesc* __restrict__ secs,
+                                                   const float* __restrict__ in4,
+                                                   const int* __restrict__ target,
+                                                   const float* __restrict__ weight,
+                                                   float* __restrict__ S) {
+    const SecDesc sd = secs[0];
+    const int r = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= sd.dim_i) return;
+    const float* src = in4 + sd.img + (size_t)sd.type * sd.N;
+    S[(size_t)r * sd.dim_i + j] = skew_cell(src, target[sd.tab + j], weight[sd.tab + j], r, j, sd.dim_i, sd.dim_o);
+}
+
+hipError_t launch_skew_full(const DevPlan& p, const SecDesc& h, const float* in4, float* S, hipStream_t s) {
+    dim3 grid((h.dim_i + 255) / 256, h.H);
+    hipLaunchKernelGGL(k_skew_full, grid, dim3(256), 0, s, p.secs, in4, p.target, p.weight, S);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// identity operator (src/kernelUNIT.h:16-23): SO = S on [first, last).
+// One wave per row work item, four rows per block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_op_unit(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
+                                                 int nwork, const float* __restrict__ in4,
+                                                 const int* __restrict__ target, const float* __restrict__ weight,
+                                                 const uint32_t* __restrict__ fl, const float* __restrict__ Sbuf,
+                                                 float* __restrict__ arena) {
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= nwork) return;
+    const int lane = threadIdx.x & 63;
+    const RowWork w = work[item];
+    const SecDesc sd = secs[w.sec];
+    const uint32_t v = fl[sd.rows + w.row];
+    const int f = fl_first(v), l = fl_last(v);
+    const float* src = in4 + sd.img + (size_t)sd.type * sd.N;
+    float* so = arena + sd.so + (size_t)w.row * sd.dim_i;
+    for (int j = f + lane; j < l; j += 64) {
+        float x;
+        if (Sbuf)
+            x = Sbuf[(size_t)w.row * sd.dim_i + j];
+        else
+            x = skew_cell(src, target[sd.tab + j], weight[sd.tab + j], w.row, j, sd.dim_i, sd.dim_o);
+        so[j] = x;
+    }
+}
+
+hipError_t launch_op_unit(const DevPlan& p, const float* in4, const float* Sbuf, float* arena, hipStream_t s) {
+    if (p.nwork == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_op_unit, dim3((p.nwork + 3) / 4), dim3(256), 0, s, p.secs, p.work, p.nwork, in4,
+                       p.target, p.weight, p.fl, Sbuf, arena);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// deskew (src/skewEngine.cuh:755-793) of all sectors of a plan, one thread per
+// oriented output cell, sectors in ascending order per orientation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float stdmaxf(float a, float b) { return (a < b) ? b : a; }
+
+// skewOutput cell (r, j) of sector index si as the deskew gather sees it.
+template <int kSource>
+__device__ __forceinline__ float so_cell(const SecDesc* __restrict__ secs, int si, const SecDesc& sd,
+                                         const uint32_t* __restrict__ fl, const float* __restrict__ src,
+                                         int r, int j) {
+    if (kSource == kDeskewDense) return src[(size_t)r * sd.dim_i + j];  // single reused buffer
+    if (r < sd.H) {
+        const uint32_t v = fl[sd.rows + r];
+        if (j < fl_first(v) || j >= fl_last(v)) return 0.0f;
+        if (kSource == kDeskewArena) return src[sd.so + (size_t)r * sd.dim_i + j];
+        return src[sd.q + r];
+    }
+    // r == H happens only when newAngle == 0 (weight 0).  The reference reads
+    // the row the previously processed sector left in its reused buffer.
+    if (kSource == kDeskewArena && sd.stale >= 0) {
+        const SecDesc pv = secs[sd.stale];
+        if (r < pv.H && pv.dim_i == sd.dim_i) {
+            const uint32_t v = fl[pv.rows + r];
+            if (j >= fl_first(v) && j < fl_last(v)) return src[pv.so + (size_t)r * sd.dim_i + j];
+        }
+    }
+    return 0.0f;
+}
+
+template <int kSource, int kMax>
+__global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs, const int* __restrict__ target,
+                                                const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                                                const float* __restrict__ src, int4 rlo, int4 rhi,
+                                                float* __restrict__ acc, int dimx, int dimy) {
+    const int t = blockIdx.z;
+    const int sb = (t == 0) ? rlo.x : (t == 1) ? rlo.z : (t == 2) ? rhi.x : rhi.z;
+    const int se = (t == 0) ? rlo.y : (t == 1) ? rlo.w : (t == 2) ? rhi.y : rhi.w;
+    if (sb >= se) return;
+    const bool tr = (t == 1 || t == 2);
+    const int dim_i = tr ? dimy : dimx, dim_o = tr ? dimx : dimy;
+    const int i = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim_o || j >= dim_i) return;
+    const size_t N = (size_t)dimx * dimy;
+    float* o = acc + (size_t)t * N + (size_t)i * dim_i + j;
+    float O = *o;
+    for (int si = sb; si < se; ++si) {
+        const SecDesc sd = secs[si];
+        const int tj = target[sd.tab + j];
+        const float wj = weight[sd.tab + j];
+        const float a = so_cell<kSource>(secs, si, sd, fl, src, i + tj, j);
+        const float b = so_cell<kSource>(secs, si, sd, fl, src, i + tj + 1, j);
+        if (kMax) {
+            O = stdmaxf(O, stdmaxf(a, b));
+            O = (float)(sd.angle + 1000 * (int)(O / 1000));
+        } else {
+            O += (1.0f - wj) * a + wj * b;
+        }
+    }
+    *o = O;
+}
+
+hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source, int is_max,
+                         float* acc, int dimx, int dimy, long long img_acc, hipStream_t s) {
+    const int dl = dimx > dimy ? dimx : dimy;
+    dim3 grid((dl + 255) / 256, dl, 4);
+    int4 lo = make_int4(range[0], range[1], range[2], range[3]);
+    int4 hi = make_int4(range[4], range[5], range[6], range[7]);
+    float* a = acc + img_acc;
+#define SKE_DESKEW(SRC, MX)                                                                              \
+    hipLaunchKernelGGL((k_deskew<SRC, MX>), grid, dim3(256), 0, s, p.secs, p.target, p.weight, p.fl, src, \
+                       lo, hi, a, dimx, dimy)
+    if (source == kDeskewDense) {
+        if (is_max) SKE_DESKEW(kDeskewDense, 1); else SKE_DESKEW(kDeskewDense, 0);
+    } else if (source == kDeskewArena) {
+        if (is_max) SKE_DESKEW(kDeskewArena, 1); else SKE_DESKEW(kDeskewArena, 0);
+    } else {
+        SKE_DESKEW(kDeskewRadon, 0);
+    }
+#undef SKE_DESKEW
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// reduce (src/skewEngine.cuh:885-912).  out[i,j] (+)= o0[i,j], o1[cj,ci],
+// o2[j,ci], o3[i,cj] in that order.  o1/o2 are read through an LDS tile so
+// that their transposed accesses stay coalesced.
+// ---------------------------------------------------------------------------
+template <int kMax>
+__global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ acc, int4 present,
+                                                float* __restrict__ out, int dimx, int dimy) {
+    __shared__ float t1[32][33];
+    __shared__ float t2[32][33];
+    const size_t N = (size_t)dimx * dimy;
+    const float* o0 = acc;
+    const float* o1 = acc + N;
+    const float* o2 = acc + 2 * N;
+    const float* o3 = acc + 3 * N;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // o1 rows cj = dimx-1-j (j in [x0, x0+32)), columns ci = dimy-1-i;
+    // o2 rows j, columns ci.  Load both tiles with ci along threadIdx.x.
+    for (int k = ty; k < 32; k += 8) {
+        const int j = x0 + k;
+        const int i = y0 + 31 - tx;  // ci = dimy-1-i runs forward with tx
+        if (j < dimx && i >= 0 && i < dimy) {
+            const int ci = dimy - 1 - i, cj = dimx - 1 - j;
+            if (present.y) t1[k][31 - tx] = o1[(size_t)cj * dimy + ci];
+            if (present.z) t2[k][31 - tx] = o2[(size_t)j * dimy + ci];
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int i = y0 + k, j = x0 + tx;
+        if (i >= dimy || j >= dimx) continue;
+        const size_t oi = (size_t)i * dimx + j;
+        float v = out[oi];
+        const int cj = dimx - 1 - j;
+        if (kMax) {
+            if (present.x) { const float u = o0[oi]; v = (v > u) ? v : u; }
+            if (present.y) { const float u = t1[tx][k]; v = (v > u) ? v : u; }
+            if (present.z) { const float u = t2[tx][k]; v = (v > u) ? v : u; }
+            if (present.w) { const float u = o3[(size_t)i * dimx + cj]; v = (v > u) ? v : u; }
+        } else {
+            if (present.x) v += o0[oi];
+            if (present.y) v += t1[tx][k];
+            if (present.z) v += t2[tx][k];
+            if (present.w) v += o3[(size_t)i * dimx + cj];
+        }
+        out[oi] = v;
+    }
+}
+
+hipError_t launch_reduce(const float* acc, const int present[4], float* out, int dimx, int dimy, int is_max,
+                         hipStream_t s) {
+    dim3 grid((dimx + 31) / 32, (dimy + 31) / 32);
+    int4 pr = make_int4(present[0], present[1], present[2], present[3]);
+    if (is_max)
+        hipLaunchKernelGGL(k_reduce<1>, grid, dim3(256), 0, s, acc, pr, out, dimx, dimy);
+    else
+        hipLaunchKernelGGL(k_reduce<0>, grid, dim3(256), 0, s, acc, pr, out, dimx, dimy);
+    return hipGetLastError();
+}
+
+}  // namespace ske

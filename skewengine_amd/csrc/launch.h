@@ -1,0 +1,64 @@
+// launch.h -- host-side launchers of the gfx950 kernels (kernels_*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ske_types.h"
+
+namespace ske {
+
+// Device view of a planned set of sectors (one image or a batch).
+struct DevPlan {
+    const SecDesc* secs = nullptr;
+    int nsec = 0;
+    const RowWork* work = nullptr;  // rows with at least one valid cell
+    int nwork = 0;
+    const int* target = nullptr;
+    const float* weight = nullptr;
+    const uint32_t* fl = nullptr;  // first | last << 16 per row
+    int maxL = 0;                   // longest valid row (LDS sizing)
+};
+
+enum DeskewSource { kDeskewDense = 0, kDeskewArena = 1, kDeskewRadon = 2 };
+
+// prepare(): the three mirrored/transposed copies (src/skewEngine.cuh:962-987).
+hipError_t launch_prepare(const float* in0, float* in1, float* in2, float* in3, int dimx, int dimy,
+                          hipStream_t s);
+
+// Materialise the skewInput rows (H*dim_i) of one sector; cells without a
+// tap are zero, as after resetSkew + skew (src/skewEngine.cuh:567,612-621).
+hipError_t launch_skew_full(const DevPlan& p, const SecDesc& h, const float* in4, float* S,
+                            hipStream_t s);
+
+// Operators.  `Sbuf` == nullptr gathers rows from the oriented inputs
+// (fused path); otherwise rows come from a materialised skewInput.
+hipError_t launch_op_unit(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
+                          hipStream_t s);
+hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
+                          const float4* rtab, float povh, int variant, hipStream_t s);
+// tab: cos/sin(2 pi t/64) interleaved (128 doubles) then cos(2 pi t/32) (32).
+hipError_t launch_op_blur(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
+                          const double* tab, hipStream_t s);
+hipError_t launch_radn_rowsum(const DevPlan& p, const float* in4, const float* Sbuf, double* P,
+                              hipStream_t s);
+// Per-sector ramp filter of the row sums in LDS (FFT length H2 <= 2^15).
+hipError_t launch_radn_filter(const DevPlan& p, const double* P, float* q, const double* tw,
+                              const int* tw_off, int maxH2, hipStream_t s);
+
+// q -> skewOutput rows on [first, last) (materialised per-sector path).
+hipError_t launch_radn_smear(const DevPlan& p, const float* q, float* SO, hipStream_t s);
+
+// Deskew sectors [first of each type ... ] of a plan into the four
+// orientation accumulators acc[4*N] (src/skewEngine.cuh:755-793), in
+// ascending sector order per orientation.  range[t] = {begin, end} indices
+// into p.secs of the sectors of type t.
+hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source,
+                         int is_max, float* acc, int dimx, int dimy, long long img_acc,
+                         hipStream_t s);
+
+// reduce(): out (+)= o0, o1, o2, o3 with the inverse orientation maps
+// (src/skewEngine.cuh:885-912).  present[t] == 0 skips buffer t.
+hipError_t launch_reduce(const float* acc, const int present[4], float* out, int dimx, int dimy,
+                         int is_max, hipStream_t s);
+
+}  // namespace ske

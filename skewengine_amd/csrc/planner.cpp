@@ -1,0 +1,183 @@
+// planner.cpp -- host-side sector planning (see planner.h).
+//
+// The maps depend on libm tan/cos in double, so they are computed here on the
+// host, exactly as src/skewEngine.cuh:503-596 computes them, and uploaded.
+#include "planner.h"
+
+#include <math.h>
+
+#include <algorithm>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+namespace ske {
+
+float fangle(int dimx, int dimy) { return (float)(atan2((double)dimy, (double)dimx) * 180 / M_PI); }
+
+void plan_sector(int dimx, int dimy, int angle, SectorPlan& sp) {
+    const double torads = M_PI / 180.0;  // src/skewEngine.cuh:258
+    const int fA = (int)fangle(dimx, dimy);
+    sp.dim_l = std::max(dimx, dimy);
+    sp.angle = angle;
+    // sector type and folded angle (:507-512)
+    sp.isT = angle > fA && angle < 180 - fA;
+    sp.isM = angle > 90;
+    if (!sp.isT && !sp.isM) { sp.type = 0; sp.newAngle = angle; }
+    if (sp.isT && !sp.isM) { sp.type = 1; sp.newAngle = 90 - angle; }
+    if (sp.isT && sp.isM) { sp.type = 2; sp.newAngle = angle - 90; }
+    if (!sp.isT && sp.isM) { sp.type = 3; sp.newAngle = 180 - angle; }
+    sp.scale = 1 / cos(torads * sp.newAngle);      // :516
+    sp.skewness = tan(sp.newAngle * torads);       // :522
+    sp.dim_i = sp.isT ? dimy : dimx;               // :525-526
+    sp.dim_o = sp.isT ? dimx : dimy;
+    sp.offset = (int)(sp.skewness * (sp.dim_i - 1));             // :528
+    sp.H = sp.newAngle ? sp.dim_o + sp.offset + 1 : sp.dim_o;    // :529
+    sp.skewWidth = sp.dim_i;
+    sp.target.assign(sp.dim_l, 0);
+    sp.weight.assign(sp.dim_l, 0.0f);
+    for (int j = 0; j < sp.dim_i; j++) {                          // :533-537
+        const double drift = sp.skewness * j;
+        sp.target[j] = (int)drift;
+        sp.weight[j] = (float)(drift - sp.target[j]);
+    }
+    // skewShape (:580-594).  The update compares j > last[] (not j+1 > last[]).
+    sp.first.assign(2 * sp.dim_l + 1, 0);
+    sp.last.assign(2 * sp.dim_l + 1, 0);
+    for (int i = 0; i < sp.dim_o; i++) sp.first[i] = 0;
+    for (int i = sp.offset; i < sp.H; i++) sp.last[i] = (uint16_t)sp.dim_l;
+    for (int i = 0; i < sp.offset; i++) sp.last[i] = 0;
+    for (int i = sp.dim_o; i < sp.H; i++) sp.first[i] = (uint16_t)sp.dim_l;
+    for (int j = 0; j < sp.dim_i; j++) {
+        const int rows = sp.target[j];
+        if (j > sp.last[rows]) sp.last[rows] = (uint16_t)(j + 1);
+        if (j > sp.last[rows + 1]) sp.last[rows + 1] = (uint16_t)(j + 1);
+        const int rowe = sp.target[j] + sp.dim_o - 1;
+        if (j < sp.first[rowe]) sp.first[rowe] = (uint16_t)j;
+        if (j < sp.first[rowe + 1]) sp.first[rowe + 1] = (uint16_t)j;
+    }
+}
+
+void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_base, long long q_base, int order,
+                ImagePlan& ip) {
+    ip = ImagePlan();
+    ip.dimx = dimx;
+    ip.dimy = dimy;
+    ip.s0 = s0;
+    ip.s1 = s1;
+    for (int t = 0; t < 4; t++) ip.range[2 * t] = ip.range[2 * t + 1] = 0;
+    int seen[4] = {0, 0, 0, 0};
+    SectorPlan sp;
+    long long so = so_base, q = q_base;
+    std::vector<std::pair<double, RowWork>> rows;
+    for (int a = s0; a < s1; a++) {
+        plan_sector(dimx, dimy, a, sp);
+        SecDesc d{};
+        d.angle = a;
+        d.newAngle = sp.newAngle;
+        d.type = sp.type;
+        d.dim_i = sp.dim_i;
+        d.dim_o = sp.dim_o;
+        d.H = sp.H;
+        d.offset = sp.offset;
+        d.tab = (int)ip.target.size();
+        d.rows = (int)ip.fl.size();
+        d.stale = -1;
+        // The deskew's zero-weight tap of a newAngle == 0 sector reads row H,
+        // which holds what the previously processed sector left there.
+        if (sp.newAngle == 0 && a > s0) d.stale = (int)ip.secs.size() - 1;
+        d.R = (float)(sp.scale * sp.scale);  // (float)pow(scale,2), src/kernelSDEM.h:171
+        d.rscale = (float)(1 / sp.scale);
+        d.img = img;
+        d.so = so;
+        d.q = q;
+        d.N = dimx * dimy;
+        const int si = (int)ip.secs.size();
+        ip.target.insert(ip.target.end(), sp.target.begin(), sp.target.begin() + sp.dim_i);
+        ip.weight.insert(ip.weight.end(), sp.weight.begin(), sp.weight.begin() + sp.dim_i);
+        for (int r = 0; r < sp.H; r++) {
+            const int f = sp.first[r], l = sp.last[r];
+            ip.fl.push_back((uint32_t)f | ((uint32_t)l << 16));
+            const int L = l - f;
+            if (L > 0) {
+                const double c = (order == kOrderCostDesc) ? (double)L * L : (double)L;
+                rows.push_back({c, RowWork{si, r}});
+                ip.maxL = std::max(ip.maxL, L);
+                ip.cells += L;
+                ip.steps += (double)L * (L - 1);
+            }
+        }
+        ip.maxH = std::max(ip.maxH, sp.H);
+        so += (long long)sp.H * sp.dim_i;
+        q += sp.H;
+        if (!seen[sp.type]) {
+            seen[sp.type] = 1;
+            ip.range[2 * sp.type] = si;
+        }
+        ip.range[2 * sp.type + 1] = si + 1;
+        ip.secs.push_back(d);
+    }
+    std::stable_sort(rows.begin(), rows.end(),
+                     [](const std::pair<double, RowWork>& x, const std::pair<double, RowWork>& y) {
+                         return x.first > y.first;
+                     });
+    ip.work.reserve(rows.size());
+    for (auto& pr : rows) ip.work.push_back(pr.second);
+    ip.so_total = so - so_base;
+    ip.q_total = q - q_base;
+}
+
+void recip_table(int n, std::vector<float>& tab4) {
+    tab4.assign(4 * (size_t)n, 0.0f);
+    for (int d = 1; d < n; d++) {
+        const float yh = (float)(1.0 / d);
+        // 1/d - yh: d*yh is exact in double and 1 - d*yh is exact (Sterbenz),
+        // so the only roundings are the final double division and the float.
+        const double res = (1.0 - (double)d * (double)yh) / d;
+        tab4[4 * d + 0] = yh;
+        tab4[4 * d + 1] = (float)res;
+        tab4[4 * d + 2] = (float)d;
+        tab4[4 * d + 3] = (float)((double)d * d);
+    }
+}
+
+void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off) {
+    tw.clear();
+    off.assign(mmax + 1, 0);
+    for (int m = 1; m <= mmax; m++) {
+        const unsigned N = 1u << m;
+        off[m] = (int)(tw.size() / 2);
+        // src/helper.h:63-81: phiT starts at e^{-i pi/N} and is squared once
+        // per stage; T_l = phiT^l by repeated products.  (ac-bd, ad+bc) with
+        // every product rounded, as std::complex<double> multiplies.
+        double thetaT = 3.14159265358979323846264338328L / N;
+        double pr = cos(thetaT), pi = -sin(thetaT);
+        unsigned k = N;
+        while (k > 1) {
+            k >>= 1;
+            double ac = pr * pr, bd = pi * pi, ad = pr * pi, bc = pi * pr;
+            pr = ac - bd;
+            pi = ad + bc;
+            double Tr = 1.0, Ti = 0.0;
+            for (unsigned l = 0; l < k; l++) {
+                tw.push_back(Tr);
+                tw.push_back(Ti);
+                ac = Tr * pr; bd = Ti * pi; ad = Tr * pi; bc = Ti * pr;
+                Tr = ac - bd;
+                Ti = ad + bc;
+            }
+        }
+    }
+}
+
+void cepstrum_table(std::vector<double>& tab) {
+    tab.assign(160, 0.0);
+    for (int t = 0; t < 64; t++) {
+        tab[2 * t] = cos(2.0 * M_PI * t / 64.0);
+        tab[2 * t + 1] = sin(2.0 * M_PI * t / 64.0);
+    }
+    for (int t = 0; t < 32; t++) tab[128 + t] = cos(2.0 * M_PI * t / 32.0);
+}
+
+}  // namespace ske

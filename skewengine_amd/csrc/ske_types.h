@@ -1,0 +1,43 @@
+// ske_types.h -- plain structs shared by the host planner and the gfx950
+// kernels.  No HIP or torch types: this header is included by both sides.
+#pragma once
+#include <stdint.h>
+
+namespace ske {
+
+// Largest grid side the engine accepts.  first/last are uint16
+// (src/skewEngine.cuh:360-361), and the row-staged kernels keep one skewed
+// row plus two 64-float guard bands in LDS.
+constexpr int kMaxDim = 16000;
+constexpr int kPad = 64;        // LDS guard band on each side of a staged row
+constexpr int kNumSectors = 180; // maxSector, src/global.cpp:69
+
+// One sector of one image, as the device sees it.  Mirrors the public fields
+// that computeParams sets (src/skewEngine.cuh:503-539).
+struct SecDesc {
+    int angle;     // a
+    int newAngle;  // folded angle
+    int type;      // sectorType 0..3, selects the oriented input copy
+    int dim_i;     // inner (row) dimension of the skewed layout
+    int dim_o;     // outer dimension
+    int H;         // skewHeight
+    int offset;
+    int tab;       // element offset of this sector's target[]/weight[] (dim_i entries)
+    int rows;      // element offset of this sector's packed first|last<<16 (H entries)
+    int stale;     // index of the sector processed just before this one when newAngle==0 (else -1)
+    float R;       // (float)(scale*scale): the viewshed area factor
+    float rscale;  // (float)(1/scale)
+    long long img; // element offset of the image's four oriented copies (4*N floats)
+    long long so;  // element offset of this sector's skewOutput rows (H*dim_i floats) in the arena
+    long long q;   // element offset of this sector's filtered projection (Radon), H floats
+    int N;         // dimx*dimy of the image
+    int pad0;
+};
+
+// One row of one sector: the unit of work of the row kernels.
+struct RowWork {
+    int sec;   // index into the SecDesc array
+    int row;   // skewed row r < H
+};
+
+}  // namespace ske

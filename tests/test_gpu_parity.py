@@ -1,0 +1,200 @@
+"""GPU parity: libskewengine.so (through its C ABI) against the CPU oracle.
+
+Bars: index maps, skewed rows, viewshed and identity outputs bit-exact
+against a one-thread oracle run (same accumulation order); Radon within
+1e-6 of the max magnitude (both compute in FP64); cepstrum per-sector values
+within 1e-4 relative.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+import skewengine_amd as ske
+from skewengine_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SECTORS = [0, 1, 10, 44, 45, 46, 60, 89, 90, 91, 120, 134, 135, 150, 179]
+
+
+@pytest.fixture(scope="module")
+def dem64():
+    return synth.dem(64, 0.8, 11, 1500.0)
+
+
+@pytest.fixture(scope="module")
+def img64():
+    return synth.blurred_texture(n=64, seed=5)[0]
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
+
+
+@pytest.mark.parametrize("kernel", [oracle.SDEM, oracle.IDENTITYDEM, oracle.RADON, oracle.CEPSTRUM])
+def test_per_sector_api_matches_oracle(kernel, dem64, img64):
+    grid = img64 if kernel in (oracle.RADON, oracle.CEPSTRUM) else dem64
+    with ske.SkewEngine(grid) as e:
+        e.set_kernel(kernel, 0.15)
+        for a in SECTORS:
+            e.skew(a)
+            e.kernel()
+            ref = oracle.sector(kernel, grid, a, povh=0.15)
+            p = e.params()
+            rp = ref["p"]
+            assert (p.a, p.newAngle, p.sectorType, p.dim_i, p.dim_o, p.offset, p.skewHeight) == (
+                rp.a, rp.newAngle, rp.sectorType, rp.dim_i, rp.dim_o, rp.offset, rp.skewHeight)
+            assert p.scale == rp.scale and p.skewness == rp.skewness
+            t, w, f, l = e.tables()
+            H, di = p.skewHeight, p.dim_i
+            assert np.array_equal(t, ref["target"][:di])
+            assert np.array_equal(w, ref["weight"][:di])
+            assert np.array_equal(f, ref["first"][:H]) and np.array_equal(l, ref["last"][:H])
+            S, SO = e.skew_buffers()
+            rS = ref["S"][:H * di].reshape(H, di)
+            rSO = ref["SO"][:H * di].reshape(H, di)
+            assert np.array_equal(S, rS), a
+            if kernel in (oracle.SDEM, oracle.IDENTITYDEM):
+                assert np.array_equal(SO, rSO), a
+            elif kernel == oracle.RADON:
+                assert _rel(SO, rSO) <= 1e-6, a
+            else:
+                bad = ~np.isclose(SO, rSO, rtol=1e-4, atol=1e-3)
+                assert bad.mean() <= 1e-3, (a, bad.sum())
+
+
+@pytest.mark.parametrize("kernel", [oracle.SDEM, oracle.IDENTITYDEM, oracle.RADON, oracle.CEPSTRUM])
+def test_per_sector_driver_matches_oracle_execute(kernel, dem64, img64):
+    """skew/kernel/deskew per sector then reduce == one-thread oracle execute()."""
+    grid = img64 if kernel in (oracle.RADON, oracle.CEPSTRUM) else dem64
+    is_max = 1 if kernel == oracle.CEPSTRUM else 0
+    with ske.SkewEngine(grid) as e:
+        e.set_kernel(kernel, 0.15)
+        for a in range(180):
+            e.skew(a)
+            e.kernel()
+            e.deskew(is_max)
+        out = np.zeros(grid.size, np.float32)
+        e.reduce(out, is_max)
+    ref, _ = oracle.execute(kernel, grid, nthreads=1, povh=0.15)
+    out = out.reshape(grid.shape)
+    if kernel in (oracle.SDEM, oracle.IDENTITYDEM):
+        assert np.array_equal(out, ref)
+    elif kernel == oracle.RADON:
+        assert _rel(out, ref) <= 1e-5
+    else:
+        assert np.mean(out != ref) <= 1e-3
+
+
+@pytest.mark.parametrize("n", [64, 128])
+@pytest.mark.parametrize("kernel", [oracle.SDEM, oracle.IDENTITYDEM, oracle.IDENTITYIMG])
+def test_run_all_sectors_bit_exact(kernel, n):
+    grid = synth.dem(n, 0.8, n, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        out = e.run(kernel, povh=0.15)
+    ref, _ = oracle.execute(kernel, grid, nthreads=1, povh=0.15)
+    assert np.array_equal(out, ref)
+
+
+def test_run_partial_sector_ranges():
+    grid = synth.dem(96, 0.75, 3, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        for s0, s1 in [(0, 1), (45, 46), (30, 100), (90, 91), (170, 180)]:
+            out = e.run(ske.KERNEL_SDEM, s0, s1)
+            ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=1, s0=s0, s1=s1)
+            assert np.array_equal(out, ref), (s0, s1)
+
+
+def test_chunked_arena_matches_single_chunk():
+    grid = synth.dem(128, 0.8, 8, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        full = e.run(ske.KERNEL_SDEM)
+        e.set_option(2, 1 << 20)  # 1 MiB arena: many chunks
+        chunked = e.run(ske.KERNEL_SDEM)
+    assert np.array_equal(full, chunked)
+
+
+def test_viewshed_c1_sizes_against_multithread_oracle():
+    """500^2 fBm DEM (H=0.8, 0-1500 m, 10 m, POVh 0.15): totals within 1e-5 of
+    the multi-threaded reference farm; both division variants bit-identical."""
+    grid = synth.dem(500, 0.8, 1, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        fast = e.run(ske.KERNEL_SDEM)
+        e.set_option(1, 1)
+        ieee = e.run(ske.KERNEL_SDEM)
+    assert np.array_equal(fast, ieee)
+    ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=0)
+    assert _rel(fast, ref) <= 1e-5
+
+
+def test_viewshed_bit_exact_sectors_at_500():
+    grid = synth.dem(500, 0.8, 1, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        e.set_kernel(ske.KERNEL_SDEM, ske.POVH_DEFAULT)
+        for a in (3, 45, 77, 133, 178):
+            e.skew(a)
+            e.kernel()
+            S, SO = e.skew_buffers()
+            ref = oracle.sector(oracle.SDEM, grid, a)
+            p = e.params()
+            assert np.array_equal(SO, ref["SO"][:p.skewHeight * p.dim_i].reshape(p.skewHeight, p.dim_i)), a
+
+
+def test_radon_c2_phantom():
+    img = synth.ellipse_phantom(512, seed=2)
+    with ske.SkewEngine(img) as e:
+        out = e.run(ske.KERNEL_RADON)
+    ref, _ = oracle.execute(oracle.RADON, img, nthreads=0)
+    assert _rel(out, ref) <= 1e-4
+
+
+def test_cepstrum_full_map():
+    img = synth.blurred_texture(n=128, seed=9)[0]
+    with ske.SkewEngine(img) as e:
+        out = e.run(ske.KERNEL_CEPSTRUM)
+    ref, _ = oracle.execute(oracle.CEPSTRUM, img, nthreads=1)
+    assert np.mean(out != ref) <= 1e-3
+    # quantised to multiples of 1000 plus the angle of the last sector of the type
+    assert np.all(np.mod(out, 1000) <= 179)
+
+
+def test_execute_and_run_entry_points():
+    grid = synth.dem(64, 0.8, 2, 1500.0)
+    out = ske.execute(ske.KERNEL_SDEM, grid)
+    ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=1)
+    assert np.array_equal(out, ref)
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "dem.flt")
+        fout = os.path.join(d, "out.flt")
+        grid.tofile(fin)
+        assert ske.run_cli(["--kernel", "sdem", "--input", fin, "--output", fout]) == 0
+        got = np.fromfile(fout, np.float32).reshape(grid.shape)
+        assert np.array_equal(got, ref)
+        # BIL int16 input: heights (float)num/10 (src/kernelSDEM.h:47)
+        raw = (np.arange(64 * 64) % 700).astype(np.int16).reshape(64, 64)
+        fbil = os.path.join(d, "dem.bil")
+        raw.tofile(fbil)
+        assert ske.run_cli(["--kernel", "0", "--input", fbil, "--output", fout]) == 0
+        got = np.fromfile(fout, np.float32).reshape(64, 64)
+        heights = (raw.astype(np.float32).astype(np.float64) / 10.0).astype(np.float32)
+        ref2, _ = oracle.execute(oracle.SDEM, heights, nthreads=1)
+        assert np.array_equal(got, ref2)
+        assert ske.run_cli([]) == 0
+
+
+def test_error_paths():
+    grid = synth.dem(32, 0.8, 2, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        with pytest.raises(ske.SkeError):
+            e.kernel()  # before skew()
+        with pytest.raises(ske.SkeError):
+            e.skew(180)
+        with pytest.raises(ske.SkeError):
+            e.run(7)
+        with pytest.raises(ske.SkeError):
+            e.run(0, 10, 5)
+    with pytest.raises(ske.SkeError):
+        ske.SkewEngine(np.zeros((30, 40), np.float32))
