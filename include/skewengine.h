@@ -120,9 +120,15 @@ int ske_output_device(ske_engine *e, const float **d_output);
 int ske_synchronize(ske_engine *e);
 
 /* Options (no reference counterpart). */
-#define SKE_OPT_SDEM_VARIANT 1  /* 0: fma-based exact division (default), 1: IEEE divide */
+#define SKE_OPT_SDEM_VARIANT 1  /* viewshed inner loop: 0 bit-block (default; rows <= 4096 cells,
+                                   longer rows use 2), 1 IEEE divide per step, 2 exact fma division
+                                   per step.  All three give identical results. */
 #define SKE_OPT_ARENA_BYTES 2   /* device budget for one chunk of skewOutput rows */
 #define SKE_OPT_PROFILE 3       /* 1: time every launch with HIP events on the engine stream */
+#define SKE_OPT_GRAPH 4         /* 1: capture the all-sector run once in a hipGraph and replay it */
+#define SKE_OPT_SDEM_WAVES 5    /* viewshed: waves per workgroup (2, 4, 8) */
+#define SKE_OPT_SDEM_PERSISTENT 6 /* viewshed: 0 one workgroup per row, k>0 persistent grid of k per CU */
+#define SKE_OPT_SDEM_OCCUPANCY 7  /* viewshed persistent grid: 0 default, 8 compiled for 8 waves/SIMD */
 int ske_set_option(ske_engine *e, int option, long long value);
 /* Accumulated HIP-event time of the launches named `name` since the last
  * reset (profiling option on): total milliseconds and launch count. */
@@ -132,6 +138,35 @@ int ske_profile_reset(ske_engine *e);
 /* Work counts of the exact maps for sectors [s0,s1) of a dimx x dimy grid:
  * cells = sum of row lengths, steps = sum L(L-1) (viewshed ray steps). */
 int ske_work(int dimx, int dimy, int s0, int s1, double *cells, double *steps);
+
+/* ---- Batches: many square grids of uneven sizes in one call.  No reference
+ * counterpart (the reference runs one grid per process); each grid's result
+ * equals what ske_run gives for it alone. ---- */
+typedef struct ske_batch ske_batch;
+/* sides[i] = dimx = dimy of grid i; inputs[i] = host (or, for the _device
+ * variant, device) pointer to sides[i]^2 floats.  Inputs are copied. */
+int ske_batch_create(int count, const int *sides, const float *const *inputs, int device_id, ske_batch **out);
+int ske_batch_create_device(int count, const int *sides, const float *const *d_inputs, int device_id,
+                            ske_batch **out);
+int ske_batch_destroy(ske_batch *b);
+/* Sectors [sector_begin, sector_end) of every grid: one operator launch per
+ * arena chunk of (grid, sector) pairs (a persistent grid for the viewshed),
+ * one deskew per grid and chunk, one reduce per grid; with SKE_OPT_GRAPH the
+ * sequence is captured once and replayed.  Asynchronous: results are ready
+ * after ske_batch_synchronize / ske_batch_output. */
+int ske_batch_run(ske_batch *b, int kernel_id, int sector_begin, int sector_end, float povh);
+int ske_batch_output(ske_batch *b, int index, float *output);
+int ske_batch_output_device(ske_batch *b, int index, const float **d_output);
+int ske_batch_synchronize(ske_batch *b);
+int ske_batch_set_option(ske_batch *b, int option, long long value);
+int ske_batch_profile_get(ske_batch *b, const char *name, double *total_ms, int *count);
+int ske_batch_profile_reset(ske_batch *b);
+/* Autotuner of the viewshed launch shape: times candidate (waves per
+ * workgroup, persistent workgroups per CU, occupancy hint) triples on a
+ * sample of the batch (one grid per distinct side, every 12th sector), keeps
+ * the fastest and returns it. */
+int ske_batch_autotune(ske_batch *b, int kernel_id, float povh, int *waves, int *wgs_per_cu, int *occupancy,
+                       double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(_HERE, "libskewengine.so")
 
 SKE_OK = 0
 KERNEL_SDEM, KERNEL_IDENTITYDEM, KERNEL_RADON, KERNEL_IDENTITYIMG, KERNEL_CEPSTRUM = 0, 1, 2, 3, 4
-OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE = 1, 2, 3
+OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE, OPT_GRAPH, OPT_SDEM_WAVES, OPT_SDEM_PERSISTENT = 1, 2, 3, 4, 5, 6
+OPT_SDEM_OCCUPANCY = 7
 MAX_SECTOR = 180
 
 # Every symbol include/skewengine.h declares.
@@ -21,6 +22,9 @@ EXPORTS = [
     "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_get_skew",
     "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option",
     "ske_profile_get", "ske_profile_reset", "ske_work",
+    "ske_batch_create", "ske_batch_create_device", "ske_batch_destroy", "ske_batch_run", "ske_batch_output",
+    "ske_batch_output_device", "ske_batch_synchronize", "ske_batch_set_option", "ske_batch_profile_get",
+    "ske_batch_profile_reset", "ske_batch_autotune",
 ]
 
 
@@ -82,6 +86,17 @@ def lib():
     L.ske_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_profile_reset.argtypes = [vp]
     L.ske_work.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p, f64p]
+    L.ske_batch_create.argtypes = [ctypes.c_int, i32p, P(vp), ctypes.c_int, P(vp)]
+    L.ske_batch_create_device.argtypes = [ctypes.c_int, i32p, P(vp), ctypes.c_int, P(vp)]
+    L.ske_batch_destroy.argtypes = [vp]
+    L.ske_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.ske_batch_output.argtypes = [vp, ctypes.c_int, vp]
+    L.ske_batch_output_device.argtypes = [vp, ctypes.c_int, P(vp)]
+    L.ske_batch_synchronize.argtypes = [vp]
+    L.ske_batch_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.ske_batch_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
+    L.ske_batch_profile_reset.argtypes = [vp]
+    L.ske_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_float, i32p, i32p, i32p, f64p]
     _lib = L
     return L
 

@@ -30,16 +30,16 @@ __device__ __forceinline__ int fl_first(uint32_t v) { return (int)(v & 0xffffu);
 __device__ __forceinline__ int fl_last(uint32_t v) { return (int)(v >> 16); }
 
 // Stage skewed row r of sector sd into LDS: row[jj] = S(r, f+jj) for
-// jj in [0, L) and zeros in the kPad guard bands on both sides.  With
+// jj in [0, L) and zeros in the `pad`-float guard bands on both sides.  With
 // Sbuf == nullptr the row is gathered from the oriented input; otherwise it
 // is read from a materialised skewInput buffer (row pitch dim_i).
 __device__ __forceinline__ void stage_row(float* row, const SecDesc& sd, int r, int f, int L,
                                           const float* __restrict__ in4,
                                           const int* __restrict__ target,
                                           const float* __restrict__ weight,
-                                          const float* __restrict__ Sbuf) {
+                                          const float* __restrict__ Sbuf, int pad = kPad) {
     const float* src = in4 + sd.img + (size_t)sd.type * sd.N;
-    for (int jj = (int)threadIdx.x - kPad; jj < L + kPad; jj += (int)blockDim.x) {
+    for (int jj = (int)threadIdx.x - pad; jj < L + pad; jj += (int)blockDim.x) {
         float v = 0.0f;
         if (jj >= 0 && jj < L) {
             const int j = f + jj;

@@ -1,16 +1,24 @@
 // engine.cpp -- the skewEngine object on one MI355X and the C ABI of
 // include/skewengine.h.
 //
-// Two paths share the kernels:
-//  * the per-sector path (ske_skew / ske_kernel / ske_deskew / ske_reduce)
-//    mirrors the reference methods one to one, with materialised
-//    skewInput/skewOutput buffers reused across sectors like the reference's;
-//  * the all-sector path (ske_run) plans every sector once, runs the operator
-//    over all rows of a chunk of sectors in one launch (rows gathered straight
-//    from the oriented inputs, results in a per-sector arena), deskews the
-//    chunk in ascending sector order and reduces on the device.
-// Both give bit-identical results to a one-thread run of the reference CPU
-// path for SDEM/identity, and match it to FP64 rounding for Radon/cepstrum.
+// Core holds the device state of one or more square grids (their four
+// oriented copies, orientation accumulators and outputs), the constant
+// tables, the planned chunks of (grid, sector) work and the skewOutput
+// arena.  Two front ends share it:
+//  * ske_engine: one grid.  Its per-sector path (ske_skew / ske_kernel /
+//    ske_deskew / ske_reduce) mirrors the reference methods one to one with
+//    materialised skewInput/skewOutput buffers reused across sectors like the
+//    reference's; ske_run goes through the all-sector path below.
+//  * ske_batch: many grids of uneven sizes in one call.
+// All-sector path: every (grid, sector) is planned once; a chunk of them that
+// fits the arena budget gets one operator launch over all its rows (rows are
+// gathered straight from the oriented inputs; the viewshed uses a persistent
+// grid pulling cost-sorted rows off an atomic counter), then one deskew per
+// grid in ascending sector order, and finally one reduce per grid.  The whole
+// sequence can be captured once in a hipGraph and replayed.
+// Results are bit-identical to a one-thread run of the reference CPU path for
+// the viewshed and identity operators, and equal to it up to FP64 rounding
+// for Radon and cepstrum.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -38,15 +46,15 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define SKE_HIP(x)                                                                               \
-    do {                                                                                         \
-        hipError_t e_ = (x);                                                                     \
+#define SKE_HIP(x)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
         if (e_ != hipSuccess) return fail(SKE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-#define SKE_TRY(x)                \
-    do {                          \
-        int rc_ = (x);            \
+#define SKE_TRY(x)                     \
+    do {                               \
+        int rc_ = (x);                 \
         if (rc_ != SKE_OK) return rc_; \
     } while (0)
 
@@ -113,10 +121,27 @@ struct DevPlanBuf {
     }
 };
 
+struct Image {
+    int n = 0;               // side (square grid)
+    long long in_off = 0;    // element offset of the four oriented copies
+    long long acc_off = 0;   // element offset of the four accumulators
+    long long out_off = 0;   // element offset of the output
+};
+
+// A chunk: consecutive (grid, sector) pairs whose skewOutput rows fit the arena.
+struct Chunk {
+    DevPlanBuf plan;
+    struct Part {
+        int img;
+        int range[8];
+    };
+    std::vector<Part> parts;
+};
+
 struct RunPlan {
     int s0 = 0, s1 = 0, order = 0;
     long long budget = 0;
-    std::vector<std::unique_ptr<DevPlanBuf>> chunks;
+    std::vector<std::unique_ptr<Chunk>> chunks;
     long long max_so = 0, max_q = 0;
     int maxH = 0;
     double cells = 0, steps = 0;
@@ -132,45 +157,71 @@ bool valid_kernel(int k) { return k >= SKE_KERNEL_SDEM && k <= SKE_KERNEL_CEPSTR
 
 constexpr int kMaxFftLog = 12;  // LDS FFT up to 4096 points (64 KB of complex doubles)
 
-}  // namespace
+int check_side(int n) {
+    if (n < 2) return fail(SKE_EINVAL, "grid must be at least 2x2");
+    if (n > kMaxDim) return fail(SKE_EUNSUPPORTED, "grid side above " + std::to_string(kMaxDim));
+    return SKE_OK;
+}
 
-struct ske_engine {
-    int dimx = 0, dimy = 0, N = 0, dim_l = 0, device = 0;
+int check_dims(int dimx, int dimy) {
+    if (dimx < 2 || dimy < 2) return fail(SKE_EINVAL, "grid must be at least 2x2");
+    if (dimx > kMaxDim || dimy > kMaxDim) return fail(SKE_EUNSUPPORTED, "grid side above " + std::to_string(kMaxDim));
+    // For dimx != dimy skewShape sets last[] = dim_l > dim_i on some rows
+    // (src/skewEngine.cuh:581), so the reference's row scans run past the
+    // row end into the next row.  Only square grids are accepted.
+    if (dimx != dimy) return fail(SKE_EUNSUPPORTED, "only square grids are supported (dimx == dimy)");
+    return SKE_OK;
+}
+
+struct Core {
+    int device = 0;
+    int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf<float> in4;   // four oriented copies
-    DevBuf<float> acc;   // four orientation accumulators
-    DevBuf<float> out;   // reduced output
-    int present[4] = {0, 0, 0, 0};
+    std::vector<Image> imgs;
+    DevBuf<float> in4, acc, out;
+    int present[4] = {0, 0, 0, 0};  // per run, same for every grid (same sector range)
     // constant tables
     DevBuf<float4> rtab;
     DevBuf<double> tw;
     DevBuf<int> tw_off;
     DevBuf<double> ctab;
+    DevBuf<int> counter;
     // options
     int sdem_variant = 0;
-    long long arena_bytes = 24LL << 30;
+    SdemTuning tune;
+    long long arena_bytes = 48LL << 30;
     bool profile = false;
+    bool use_graph = false;
+    // profiling
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int>> prof;
-    // per-sector path
-    int kernel_id = SKE_KERNEL_IDENTITYDEM;
-    float povh = 0.15f;
-    ske_device_operator custom = nullptr;
-    void* custom_user = nullptr;
-    bool have_sector = false;
-    bool have_kernel_out = false;
-    SectorPlan cur;
-    DevPlanBuf one;
-    DevBuf<float> S, SO;          // 2N + dim_l each, like skewAlloc (src/skewEngine.cuh:425-426)
-    DevBuf<uint16_t> first1, last1;
-    DevBuf<double> P1;
-    DevBuf<float> q1;
-    // all-sector path
+    bool capturing = false;
+    // planned runs
     std::unique_ptr<RunPlan> plan;
     DevBuf<float> arena;
     DevBuf<double> P;
     DevBuf<float> q;
+    // captured graph of the last all-sector run
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
+
+    ~Core() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        for (auto& t : pending) {
+            (void)hipEventDestroy(std::get<1>(t));
+            (void)hipEventDestroy(std::get<2>(t));
+        }
+        for (auto ev : event_pool) (void)hipEventDestroy(ev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
 
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -183,19 +234,20 @@ struct ske_engine {
         return ev;
     }
 
-    // Launch through `fn` (returning hipError_t); with profiling on, bracket
-    // it with HIP events on the engine stream.
+    // Launch through `fn` (returning hipError_t); with profiling on (and not
+    // capturing a graph), bracket it with HIP events on the stream.
     template <typename F>
     int launch(const char* name, F&& fn) {
+        const bool timed = profile && !capturing;
         hipEvent_t a = nullptr, b = nullptr;
-        if (profile) {
+        if (timed) {
             a = take_event();
             b = take_event();
             SKE_HIP(hipEventRecord(a, stream));
         }
         hipError_t e = fn();
         if (e != hipSuccess) return fail(SKE_EHIP, std::string(name) + ": " + hipGetErrorString(e));
-        if (profile) {
+        if (timed) {
             SKE_HIP(hipEventRecord(b, stream));
             pending.emplace_back(name, a, b);
         }
@@ -217,7 +269,14 @@ struct ske_engine {
         return SKE_OK;
     }
 
-    int init_tables() {
+    int init(int device_id) {
+        int ndev = 0;
+        SKE_HIP(hipGetDeviceCount(&ndev));
+        if (device_id < 0 || device_id >= ndev) return fail(SKE_EINVAL, "device " + std::to_string(device_id) + " not present");
+        device = device_id;
+        SKE_HIP(hipSetDevice(device));
+        SKE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        SKE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         std::vector<float> r;
         recip_table(kMaxDim + 1, r);
         SKE_TRY(rtab.upload(reinterpret_cast<const float4*>(r.data()), kMaxDim + 1, stream));
@@ -229,164 +288,342 @@ struct ske_engine {
         std::vector<double> c;
         cepstrum_table(c);
         SKE_TRY(ctab.upload(c.data(), c.size(), stream));
+        SKE_TRY(counter.ensure(16));
         SKE_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
         return SKE_OK;
     }
 
-    ~ske_engine() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& t : pending) {
-            (void)hipEventDestroy(std::get<1>(t));
-            (void)hipEventDestroy(std::get<2>(t));
+    // Upload the grids and build their oriented copies (prepare).
+    int add_images(int count, const int* sides, const float* const* inputs, bool on_device) {
+        long long tot = 0;
+        for (int i = 0; i < count; i++) {
+            SKE_TRY(check_side(sides[i]));
+            if (!inputs[i]) return fail(SKE_EINVAL, "input " + std::to_string(i) + " is NULL");
+            Image im;
+            im.n = sides[i];
+            const long long N = (long long)im.n * im.n;
+            im.in_off = 4 * tot;
+            im.acc_off = 4 * tot;
+            im.out_off = tot;
+            tot += N;
+            imgs.push_back(im);
         }
-        for (auto ev : event_pool) (void)hipEventDestroy(ev);
-        if (stream) (void)hipStreamDestroy(stream);
+        SKE_TRY(in4.ensure(4 * (size_t)tot));
+        SKE_TRY(acc.ensure(4 * (size_t)tot));
+        SKE_TRY(out.ensure((size_t)tot));
+        for (int i = 0; i < count; i++) {
+            const Image& im = imgs[i];
+            const size_t N = (size_t)im.n * im.n;
+            float* base = in4.p + im.in_off;
+            SKE_HIP(hipMemcpyAsync(base, inputs[i], N * sizeof(float),
+                                   on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+            SKE_TRY(launch("prepare", [&] {
+                return launch_prepare(base, base + N, base + 2 * N, base + 3 * N, im.n, im.n, stream);
+            }));
+        }
+        SKE_TRY(launch("fill", [&] { return launch_fill_f32(acc.p, 4 * (size_t)tot, 0.0f, stream); }));
+        SKE_TRY(launch("fill", [&] { return launch_fill_f32(out.p, (size_t)tot, 0.0f, stream); }));
+        return sync();
+    }
+
+    // Plan (or reuse) the chunks of sectors [s0,s1) of every grid.
+    int ensure_plan(int s0, int s1, int order) {
+        if (plan && plan->s0 == s0 && plan->s1 == s1 && plan->order == order && plan->budget == arena_bytes)
+            return SKE_OK;
+        drop_graph();
+        std::unique_ptr<RunPlan> rp(new RunPlan());
+        rp->s0 = s0;
+        rp->s1 = s1;
+        rp->order = order;
+        rp->budget = arena_bytes;
+        const long long budget_floats = std::max<long long>(arena_bytes / 4, 1);
+        std::unique_ptr<Chunk> cur(new Chunk());
+        SectorPlan sp;
+        auto flush = [&]() -> int {
+            if (cur->parts.empty()) return SKE_OK;
+            plan_finish(cur->plan.ip, order);
+            SKE_TRY(cur->plan.upload(stream));
+            rp->max_so = std::max(rp->max_so, cur->plan.ip.so_total);
+            rp->max_q = std::max(rp->max_q, cur->plan.ip.q_total);
+            rp->maxH = std::max(rp->maxH, cur->plan.ip.maxH);
+            rp->cells += cur->plan.ip.cells;
+            rp->steps += cur->plan.ip.steps;
+            rp->chunks.push_back(std::move(cur));
+            cur.reset(new Chunk());
+            return SKE_OK;
+        };
+        for (int gi = 0; gi < (int)imgs.size(); gi++) {
+            const int n = imgs[gi].n;
+            int a = s0;
+            while (a < s1) {
+                // extend this grid's run of sectors while it fits the chunk
+                int b = a;
+                long long have = cur->plan.ip.so_total;
+                while (b < s1) {
+                    plan_sector(n, n, b, sp);
+                    const long long need = (long long)sp.H * sp.dim_i;
+                    if (have + need > budget_floats && !(cur->parts.empty() && b == a)) break;
+                    have += need;
+                    b++;
+                }
+                if (b == a) {  // nothing fits next to what the chunk holds: start a new chunk
+                    SKE_TRY(flush());
+                    continue;
+                }
+                Chunk::Part part;
+                part.img = gi;
+                plan_append(n, n, a, b, imgs[gi].in_off, cur->plan.ip, part.range);
+                cur->parts.push_back(part);
+                a = b;
+                if (a < s1) SKE_TRY(flush());
+            }
+        }
+        SKE_TRY(flush());
+        SKE_HIP(hipStreamSynchronize(stream));
+        present[0] = present[1] = present[2] = present[3] = 0;
+        for (int a = s0; a < s1 && !imgs.empty(); a++) {
+            plan_sector(imgs[0].n, imgs[0].n, a, sp);
+            present[sp.type] = 1;
+        }
+        plan = std::move(rp);
+        return SKE_OK;
+    }
+
+    int prepare_run(int kernel_id, int s0, int s1) {
+        if (!valid_kernel(kernel_id)) return fail(SKE_EINVAL, "unknown operator id " + std::to_string(kernel_id));
+        if (s0 < 0 || s1 > SKE_MAX_SECTOR || s0 >= s1) return fail(SKE_EINVAL, "sector range must be within [0,180)");
+        if (imgs.empty()) return fail(SKE_ESTATE, "no grids");
+        SKE_HIP(hipSetDevice(device));
+        const int order = (kernel_id == SKE_KERNEL_SDEM) ? kOrderCostDesc : kOrderNatural;
+        SKE_TRY(ensure_plan(s0, s1, order));
+        if (kernel_id == SKE_KERNEL_RADON) {
+            if (next_pow2_log(plan->maxH) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
+            if (P.n < (size_t)plan->max_q || q.n < (size_t)plan->max_q) drop_graph();
+            SKE_TRY(P.ensure((size_t)plan->max_q));
+            SKE_TRY(q.ensure((size_t)plan->max_q));
+        } else {
+            if (arena.n < (size_t)plan->max_so) drop_graph();
+            SKE_TRY(arena.ensure((size_t)plan->max_so));
+        }
+        return SKE_OK;
+    }
+
+    // Enqueue the whole all-sector run (no allocation, no synchronisation:
+    // safe to capture).
+    int enqueue_run(int kernel_id, float povh) {
+        const bool radon = kernel_id == SKE_KERNEL_RADON;
+        const int is_max = kernel_id == SKE_KERNEL_CEPSTRUM ? 1 : 0;
+        const size_t acc_n = acc.n;
+        SKE_TRY(launch("fill", [&] { return launch_fill_f32(acc.p, acc_n, 0.0f, stream); }));
+        for (auto& cp : plan->chunks) {
+            const DevPlan v = cp->plan.view();
+            const ImagePlan& ip = cp->plan.ip;
+            switch (kernel_id) {
+            case SKE_KERNEL_SDEM:
+                if (sdem_variant == 0) {
+                    SKE_TRY(launch("op_sdem", [&] {
+                        return launch_op_sdem_tuned(v, in4.p, nullptr, arena.p, rtab.p, povh, tune, counter.p, stream);
+                    }));
+                } else {
+                    SKE_TRY(launch("op_sdem", [&] {
+                        return launch_op_sdem(v, in4.p, nullptr, arena.p, rtab.p, povh, sdem_variant, stream);
+                    }));
+                }
+                break;
+            case SKE_KERNEL_CEPSTRUM:
+                SKE_TRY(launch("op_blur", [&] { return launch_op_blur(v, in4.p, nullptr, arena.p, ctab.p, stream); }));
+                break;
+            case SKE_KERNEL_RADON:
+                SKE_TRY(launch("fill", [&] { return launch_fill_f64(P.p, (size_t)ip.q_total, stream); }));
+                SKE_TRY(launch("radn_rowsum", [&] { return launch_radn_rowsum(v, in4.p, nullptr, P.p, stream); }));
+                SKE_TRY(launch("radn_filter", [&] {
+                    return launch_radn_filter(v, P.p, q.p, tw.p, tw_off.p, 1 << next_pow2_log(ip.maxH), stream);
+                }));
+                break;
+            default:
+                SKE_TRY(launch("op_unit", [&] { return launch_op_unit(v, in4.p, nullptr, arena.p, stream); }));
+                break;
+            }
+            const float* src = radon ? q.p : arena.p;
+            const int source = radon ? kDeskewRadon : kDeskewArena;
+            for (const Chunk::Part& part : cp->parts) {
+                const Image& im = imgs[part.img];
+                SKE_TRY(launch("deskew", [&] {
+                    return launch_deskew(v, part.range, src, source, is_max, acc.p, im.n, im.n, im.acc_off, stream);
+                }));
+            }
+        }
+        const size_t out_n = out.n;
+        SKE_TRY(launch("fill", [&] { return launch_fill_f32(out.p, out_n, 0.0f, stream); }));
+        for (const Image& im : imgs) {
+            SKE_TRY(launch("reduce", [&] {
+                return launch_reduce(acc.p + im.acc_off, present, out.p + im.out_off, im.n, im.n, is_max, stream);
+            }));
+        }
+        return SKE_OK;
+    }
+
+    int run(int kernel_id, int s0, int s1, float povh) {
+        SKE_TRY(prepare_run(kernel_id, s0, s1));
+        if (!use_graph) return enqueue_run(kernel_id, povh);
+        char key[160];
+        unsigned pbits = 0;
+        memcpy(&pbits, &povh, sizeof pbits);
+        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%lld", kernel_id, s0, s1, pbits, sdem_variant,
+                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, arena_bytes);
+        if (!gexec || gkey != key) {
+            drop_graph();
+            SKE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            capturing = true;
+            int rc = enqueue_run(kernel_id, povh);
+            capturing = false;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc != SKE_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(SKE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                gexec = nullptr;
+                return fail(SKE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            gkey = key;
+        }
+        return launch("graph", [&] { return hipGraphLaunch(gexec, stream); });
+    }
+
+    int set_option(int option, long long value) {
+        switch (option) {
+        case SKE_OPT_SDEM_VARIANT:
+            if (value < 0 || value > 2) return fail(SKE_EINVAL, "SDEM variant must be 0, 1 or 2");
+            sdem_variant = (int)value;
+            return SKE_OK;
+        case SKE_OPT_ARENA_BYTES:
+            if (value < (1LL << 20)) return fail(SKE_EINVAL, "arena budget below 1 MiB");
+            arena_bytes = value;
+            return SKE_OK;
+        case SKE_OPT_PROFILE:
+            profile = value != 0;
+            return SKE_OK;
+        case SKE_OPT_GRAPH:
+            use_graph = value != 0;
+            if (!use_graph) drop_graph();
+            return SKE_OK;
+        case SKE_OPT_SDEM_WAVES:
+            if (value != 2 && value != 4 && value != 8) return fail(SKE_EINVAL, "waves per workgroup must be 2, 4 or 8");
+            tune.waves = (int)value;
+            return SKE_OK;
+        case SKE_OPT_SDEM_PERSISTENT:
+            if (value < 0 || value > 64) return fail(SKE_EINVAL, "persistent workgroups per CU must be in [0,64]");
+            tune.persistent_wgs_per_cu = (int)value;
+            return SKE_OK;
+        case SKE_OPT_SDEM_OCCUPANCY:
+            if (value != 0 && value != 8) return fail(SKE_EINVAL, "occupancy hint must be 0 or 8");
+            tune.occupancy = (int)value;
+            return SKE_OK;
+        default:
+            return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
+        }
+    }
+
+    int profile_get(const char* name, double* total_ms, int* count) {
+        SKE_TRY(sync());
+        auto it = prof.find(name);
+        if (total_ms) *total_ms = it == prof.end() ? 0.0 : it->second.first;
+        if (count) *count = it == prof.end() ? 0 : it->second.second;
+        return SKE_OK;
+    }
+
+    // Time candidate viewshed launch shapes on a sample (one grid per distinct
+    // side, every 12th sector) and keep the fastest.
+    int autotune_sdem(float povh, int* waves, int* wgs, int* occ, double* best_ms) {
+        SKE_HIP(hipSetDevice(device));
+        std::unique_ptr<DevPlanBuf> sample(new DevPlanBuf());
+        std::vector<int> seen;
+        for (const Image& im : imgs) {
+            if (std::find(seen.begin(), seen.end(), im.n) != seen.end()) continue;
+            seen.push_back(im.n);
+            int range[8];
+            for (int a = 0; a < SKE_MAX_SECTOR; a += 12) plan_append(im.n, im.n, a, a + 1, im.in_off, sample->ip, range);
+        }
+        plan_finish(sample->ip, kOrderCostDesc);
+        SKE_TRY(sample->upload(stream));
+        DevBuf<float> scratch;
+        SKE_TRY(scratch.ensure((size_t)sample->ip.so_total));
+        const DevPlan v = sample->view();
+        const SdemTuning cands[] = {{4, 0, 0}, {8, 0, 0}, {4, 4, 0}, {8, 4, 0}, {2, 8, 0}, {4, 8, 8},
+                                    {8, 4, 8}, {2, 16, 8}, {4, 16, 8}, {8, 8, 8}};
+        hipEvent_t a = take_event(), b = take_event();
+        double best = 1e300;
+        SdemTuning pick = tune;
+        for (const SdemTuning& c : cands) {
+            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, stream));
+            SKE_HIP(hipEventRecord(a, stream));
+            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, stream));
+            SKE_HIP(hipEventRecord(b, stream));
+            SKE_HIP(hipEventSynchronize(b));
+            float ms = 0.0f;
+            SKE_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        event_pool.push_back(a);
+        event_pool.push_back(b);
+        SKE_HIP(hipStreamSynchronize(stream));
+        tune = pick;
+        drop_graph();
+        if (waves) *waves = pick.waves;
+        if (wgs) *wgs = pick.persistent_wgs_per_cu;
+        if (occ) *occ = pick.occupancy;
+        if (best_ms) *best_ms = best;
+        return SKE_OK;
     }
 };
 
-namespace {
+}  // namespace
 
-int check_dims(int dimx, int dimy) {
-    if (dimx < 2 || dimy < 2) return fail(SKE_EINVAL, "grid must be at least 2x2");
-    if (dimx > kMaxDim || dimy > kMaxDim)
-        return fail(SKE_EUNSUPPORTED, "grid side above " + std::to_string(kMaxDim));
-    // For dimx != dimy skewShape sets last[] = dim_l > dim_i on some rows
-    // (src/skewEngine.cuh:581), so the reference's row scans run past the
-    // row end into the next row.  Only square grids are accepted.
-    if (dimx != dimy) return fail(SKE_EUNSUPPORTED, "only square grids are supported (dimx == dimy)");
-    return SKE_OK;
-}
+struct ske_batch {
+    Core core;
+};
+
+struct ske_engine {
+    Core core;
+    int dimx = 0, dimy = 0, N = 0, dim_l = 0;
+    int present[4] = {0, 0, 0, 0};  // per-sector path
+    int kernel_id = SKE_KERNEL_IDENTITYDEM;
+    float povh = 0.15f;
+    ske_device_operator custom = nullptr;
+    void* custom_user = nullptr;
+    bool have_sector = false;
+    SectorPlan cur;
+    DevPlanBuf one;
+    DevBuf<float> S, SO;  // 2N + dim_l each, like skewAlloc (src/skewEngine.cuh:425-426)
+    DevBuf<uint16_t> first1, last1;
+    DevBuf<double> P1;
+    DevBuf<float> q1;
+};
+
+namespace {
 
 int create_common(int dimx, int dimy, const float* input, bool on_device, int device_id, ske_engine** out) {
     if (!out) return fail(SKE_EINVAL, "out is NULL");
     *out = nullptr;
     if (!input) return fail(SKE_EINVAL, "input is NULL");
     SKE_TRY(check_dims(dimx, dimy));
-    int ndev = 0;
-    SKE_HIP(hipGetDeviceCount(&ndev));
-    if (device_id < 0 || device_id >= ndev)
-        return fail(SKE_EINVAL, "device " + std::to_string(device_id) + " not present");
-    SKE_HIP(hipSetDevice(device_id));
     std::unique_ptr<ske_engine> e(new ske_engine());
+    SKE_TRY(e->core.init(device_id));
     e->dimx = dimx;
     e->dimy = dimy;
     e->N = dimx * dimy;
     e->dim_l = std::max(dimx, dimy);
-    e->device = device_id;
-    SKE_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    const size_t N = (size_t)e->N;
-    SKE_TRY(e->in4.ensure(4 * N));
-    SKE_TRY(e->acc.ensure(4 * N));
-    SKE_TRY(e->out.ensure(N));
-    SKE_HIP(hipMemcpyAsync(e->in4.p, input, N * sizeof(float),
-                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, e->stream));
-    SKE_TRY(e->launch("prepare", [&] {
-        return launch_prepare(e->in4.p, e->in4.p + N, e->in4.p + 2 * N, e->in4.p + 3 * N, dimx, dimy, e->stream);
-    }));
-    SKE_HIP(hipMemsetAsync(e->acc.p, 0, 4 * N * sizeof(float), e->stream));
-    SKE_TRY(e->init_tables());
+    const int side = dimx;
+    SKE_TRY(e->core.add_images(1, &side, &input, on_device));
     *out = e.release();
-    return SKE_OK;
-}
-
-// Plan (or reuse) the chunks of sectors [s0,s1).  A chunk holds as many
-// consecutive sectors as fit the arena budget.
-int ensure_plan(ske_engine* e, int s0, int s1, int order) {
-    if (e->plan && e->plan->s0 == s0 && e->plan->s1 == s1 && e->plan->order == order &&
-        e->plan->budget == e->arena_bytes)
-        return SKE_OK;
-    std::unique_ptr<RunPlan> rp(new RunPlan());
-    rp->s0 = s0;
-    rp->s1 = s1;
-    rp->order = order;
-    rp->budget = e->arena_bytes;
-    const long long budget_floats = std::max<long long>(e->arena_bytes / 4, 1);
-    SectorPlan sp;
-    int a = s0;
-    while (a < s1) {
-        long long acc_floats = 0;
-        int b = a;
-        while (b < s1) {
-            plan_sector(e->dimx, e->dimy, b, sp);
-            const long long need = (long long)sp.H * sp.dim_i;
-            if (b > a && acc_floats + need > budget_floats) break;
-            acc_floats += need;
-            b++;
-        }
-        std::unique_ptr<DevPlanBuf> c(new DevPlanBuf());
-        plan_image(e->dimx, e->dimy, a, b, 0, 0, 0, order, c->ip);
-        SKE_TRY(c->upload(e->stream));
-        rp->max_so = std::max(rp->max_so, c->ip.so_total);
-        rp->max_q = std::max(rp->max_q, c->ip.q_total);
-        rp->maxH = std::max(rp->maxH, c->ip.maxH);
-        rp->cells += c->ip.cells;
-        rp->steps += c->ip.steps;
-        rp->chunks.push_back(std::move(c));
-        a = b;
-    }
-    SKE_HIP(hipStreamSynchronize(e->stream));
-    e->plan = std::move(rp);
-    return SKE_OK;
-}
-
-int run_all(ske_engine* e, int kernel_id, int s0, int s1, float povh) {
-    if (!valid_kernel(kernel_id)) return fail(SKE_EINVAL, "unknown operator id " + std::to_string(kernel_id));
-    if (s0 < 0 || s1 > SKE_MAX_SECTOR || s0 >= s1) return fail(SKE_EINVAL, "sector range must be within [0,180)");
-    SKE_HIP(hipSetDevice(e->device));
-    const int order = (kernel_id == SKE_KERNEL_SDEM) ? kOrderCostDesc : kOrderNatural;
-    SKE_TRY(ensure_plan(e, s0, s1, order));
-    RunPlan& rp = *e->plan;
-    const bool radon = kernel_id == SKE_KERNEL_RADON;
-    const int is_max = kernel_id == SKE_KERNEL_CEPSTRUM ? 1 : 0;
-    if (radon && next_pow2_log(rp.maxH) > kMaxFftLog)
-        return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
-    const size_t N = (size_t)e->N;
-    if (radon) {
-        SKE_TRY(e->P.ensure((size_t)rp.max_q));
-        SKE_TRY(e->q.ensure((size_t)rp.max_q));
-    } else {
-        SKE_TRY(e->arena.ensure((size_t)rp.max_so));
-    }
-    SKE_HIP(hipMemsetAsync(e->acc.p, 0, 4 * N * sizeof(float), e->stream));
-    for (int t = 0; t < 4; t++) e->present[t] = 0;
-    for (auto& cp : rp.chunks) {
-        const DevPlan v = cp->view();
-        const ImagePlan& ip = cp->ip;
-        switch (kernel_id) {
-        case SKE_KERNEL_SDEM:
-            SKE_TRY(e->launch("op_sdem", [&] {
-                return launch_op_sdem(v, e->in4.p, nullptr, e->arena.p, e->rtab.p, povh, e->sdem_variant, e->stream);
-            }));
-            break;
-        case SKE_KERNEL_CEPSTRUM:
-            SKE_TRY(e->launch("op_blur", [&] {
-                return launch_op_blur(v, e->in4.p, nullptr, e->arena.p, e->ctab.p, e->stream);
-            }));
-            break;
-        case SKE_KERNEL_RADON:
-            SKE_HIP(hipMemsetAsync(e->P.p, 0, (size_t)ip.q_total * sizeof(double), e->stream));
-            SKE_TRY(e->launch("radn_rowsum", [&] { return launch_radn_rowsum(v, e->in4.p, nullptr, e->P.p, e->stream); }));
-            SKE_TRY(e->launch("radn_filter", [&] {
-                return launch_radn_filter(v, e->P.p, e->q.p, e->tw.p, e->tw_off.p, 1 << next_pow2_log(ip.maxH),
-                                          e->stream);
-            }));
-            break;
-        default:
-            SKE_TRY(e->launch("op_unit", [&] { return launch_op_unit(v, e->in4.p, nullptr, e->arena.p, e->stream); }));
-            break;
-        }
-        const float* src = radon ? e->q.p : e->arena.p;
-        const int source = radon ? kDeskewRadon : kDeskewArena;
-        SKE_TRY(e->launch("deskew", [&] {
-            return launch_deskew(v, ip.range, src, source, is_max, e->acc.p, e->dimx, e->dimy, 0, e->stream);
-        }));
-        for (int t = 0; t < 4; t++)
-            if (ip.range[2 * t] < ip.range[2 * t + 1]) e->present[t] = 1;
-    }
-    SKE_HIP(hipMemsetAsync(e->out.p, 0, N * sizeof(float), e->stream));
-    SKE_TRY(e->launch("reduce", [&] {
-        return launch_reduce(e->acc.p, e->present, e->out.p, e->dimx, e->dimy, is_max, e->stream);
-    }));
     return SKE_OK;
 }
 
@@ -394,88 +631,89 @@ int run_all(ske_engine* e, int kernel_id, int s0, int s1, float povh) {
 
 int sector_skew(ske_engine* e, int angle) {
     if (angle < 0 || angle >= SKE_MAX_SECTOR) return fail(SKE_EINVAL, "angle must be in [0,180)");
-    SKE_HIP(hipSetDevice(e->device));
+    Core& c = e->core;
+    SKE_HIP(hipSetDevice(c.device));
     plan_sector(e->dimx, e->dimy, angle, e->cur);
     const SectorPlan& sp = e->cur;
     ImagePlan& ip = e->one.ip;
     plan_image(e->dimx, e->dimy, angle, angle + 1, 0, 0, 0, kOrderNatural, ip);
     ip.secs[0].stale = -1;  // the reused buffer carries the stale row itself
-    SKE_TRY(e->one.upload(e->stream));
+    SKE_TRY(e->one.upload(c.stream));
     const size_t sk = 2 * (size_t)e->N + e->dim_l;
     if (!e->S.p) {
         SKE_TRY(e->S.ensure(sk));
         SKE_TRY(e->SO.ensure(sk));
-        SKE_HIP(hipMemsetAsync(e->S.p, 0, sk * sizeof(float), e->stream));
-        SKE_HIP(hipMemsetAsync(e->SO.p, 0, sk * sizeof(float), e->stream));
+        SKE_HIP(hipMemsetAsync(e->S.p, 0, sk * sizeof(float), c.stream));
+        SKE_HIP(hipMemsetAsync(e->SO.p, 0, sk * sizeof(float), c.stream));
         SKE_TRY(e->first1.ensure(2 * (size_t)e->dim_l + 1));
         SKE_TRY(e->last1.ensure(2 * (size_t)e->dim_l + 1));
         SKE_TRY(e->P1.ensure(2 * (size_t)e->dim_l + 2));
         SKE_TRY(e->q1.ensure(2 * (size_t)e->dim_l + 2));
     }
     SKE_HIP(hipMemcpyAsync(e->first1.p, sp.first.data(), sp.first.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
-                           e->stream));
+                           c.stream));
     SKE_HIP(hipMemcpyAsync(e->last1.p, sp.last.data(), sp.last.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
-                           e->stream));
+                           c.stream));
     // resetSkew: zero skewHeight*dim_i of both buffers (src/skewEngine.cuh:567-568)
     const size_t cells = (size_t)sp.H * sp.dim_i;
-    SKE_HIP(hipMemsetAsync(e->S.p, 0, cells * sizeof(float), e->stream));
-    SKE_HIP(hipMemsetAsync(e->SO.p, 0, cells * sizeof(float), e->stream));
+    SKE_HIP(hipMemsetAsync(e->S.p, 0, cells * sizeof(float), c.stream));
+    SKE_HIP(hipMemsetAsync(e->SO.p, 0, cells * sizeof(float), c.stream));
     const DevPlan v = e->one.view();
-    SKE_TRY(e->launch("skew_full", [&] { return launch_skew_full(v, ip.secs[0], e->in4.p, e->S.p, e->stream); }));
-    SKE_HIP(hipStreamSynchronize(e->stream));  // host plan vectors are reused on the next call
+    SKE_TRY(c.launch("skew_full", [&] { return launch_skew_full(v, ip.secs[0], c.in4.p, e->S.p, c.stream); }));
+    SKE_HIP(hipStreamSynchronize(c.stream));  // host plan vectors are reused on the next call
     e->have_sector = true;
-    e->have_kernel_out = false;
     return SKE_OK;
 }
 
 int sector_kernel(ske_engine* e) {
     if (!e->have_sector) return fail(SKE_ESTATE, "kernel() before skew()");
-    SKE_HIP(hipSetDevice(e->device));
+    Core& c = e->core;
+    SKE_HIP(hipSetDevice(c.device));
     const DevPlan v = e->one.view();
     const SectorPlan& sp = e->cur;
     if (e->custom) {
-        e->custom(e->SO.p, e->S.p, sp.dim_i, sp.H, e->last1.p, e->first1.p, e->povh, sp.newAngle,
-                  (void*)e->stream, e->custom_user);
+        e->custom(e->SO.p, e->S.p, sp.dim_i, sp.H, e->last1.p, e->first1.p, e->povh, sp.newAngle, (void*)c.stream,
+                  e->custom_user);
         SKE_HIP(hipGetLastError());
-    } else {
-        switch (e->kernel_id) {
-        case SKE_KERNEL_SDEM:
-            SKE_TRY(e->launch("op_sdem", [&] {
-                return launch_op_sdem(v, e->in4.p, e->S.p, e->SO.p, e->rtab.p, e->povh, e->sdem_variant, e->stream);
-            }));
-            break;
-        case SKE_KERNEL_CEPSTRUM:
-            SKE_TRY(e->launch("op_blur", [&] { return launch_op_blur(v, e->in4.p, e->S.p, e->SO.p, e->ctab.p, e->stream); }));
-            break;
-        case SKE_KERNEL_RADON: {
-            if (next_pow2_log(sp.H) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
-            SKE_HIP(hipMemsetAsync(e->P1.p, 0, (size_t)sp.H * sizeof(double), e->stream));
-            SKE_TRY(e->launch("radn_rowsum", [&] { return launch_radn_rowsum(v, e->in4.p, e->S.p, e->P1.p, e->stream); }));
-            SKE_TRY(e->launch("radn_filter", [&] {
-                return launch_radn_filter(v, e->P1.p, e->q1.p, e->tw.p, e->tw_off.p, 1 << next_pow2_log(sp.H), e->stream);
-            }));
-            SKE_TRY(e->launch("radn_smear", [&] { return launch_radn_smear(v, e->q1.p, e->SO.p, e->stream); }));
-            break;
-        }
-        default:
-            SKE_TRY(e->launch("op_unit", [&] { return launch_op_unit(v, e->in4.p, e->S.p, e->SO.p, e->stream); }));
-            break;
-        }
+        return SKE_OK;
     }
-    e->have_kernel_out = true;
+    switch (e->kernel_id) {
+    case SKE_KERNEL_SDEM:
+        SKE_TRY(c.launch("op_sdem", [&] {
+            return launch_op_sdem(v, c.in4.p, e->S.p, e->SO.p, c.rtab.p, e->povh, c.sdem_variant, c.stream);
+        }));
+        break;
+    case SKE_KERNEL_CEPSTRUM:
+        SKE_TRY(c.launch("op_blur", [&] { return launch_op_blur(v, c.in4.p, e->S.p, e->SO.p, c.ctab.p, c.stream); }));
+        break;
+    case SKE_KERNEL_RADON: {
+        if (next_pow2_log(sp.H) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
+        SKE_HIP(hipMemsetAsync(e->P1.p, 0, (size_t)sp.H * sizeof(double), c.stream));
+        SKE_TRY(c.launch("radn_rowsum", [&] { return launch_radn_rowsum(v, c.in4.p, e->S.p, e->P1.p, c.stream); }));
+        SKE_TRY(c.launch("radn_filter", [&] {
+            return launch_radn_filter(v, e->P1.p, e->q1.p, c.tw.p, c.tw_off.p, 1 << next_pow2_log(sp.H), c.stream);
+        }));
+        SKE_TRY(c.launch("radn_smear", [&] { return launch_radn_smear(v, e->q1.p, e->SO.p, c.stream); }));
+        break;
+    }
+    default:
+        SKE_TRY(c.launch("op_unit", [&] { return launch_op_unit(v, c.in4.p, e->S.p, e->SO.p, c.stream); }));
+        break;
+    }
     return SKE_OK;
 }
 
 int sector_deskew(ske_engine* e, int is_max) {
     if (!e->have_sector) return fail(SKE_ESTATE, "deskew() before skew()");
-    SKE_HIP(hipSetDevice(e->device));
+    Core& c = e->core;
+    SKE_HIP(hipSetDevice(c.device));
     const DevPlan v = e->one.view();
     const int t = e->cur.type;
     int range[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     range[2 * t] = 0;
     range[2 * t + 1] = 1;
-    SKE_TRY(e->launch("deskew", [&] {
-        return launch_deskew(v, range, e->SO.p, kDeskewDense, is_max ? 1 : 0, e->acc.p, e->dimx, e->dimy, 0, e->stream);
+    SKE_TRY(c.launch("deskew", [&] {
+        return launch_deskew(v, range, e->SO.p, kDeskewDense, is_max ? 1 : 0, c.acc.p, e->dimx, e->dimy, 0, c.stream);
     }));
     e->present[t] = 1;
     return SKE_OK;
@@ -489,7 +727,7 @@ extern "C" {
 
 const char* ske_last_error(void) { return g_err.c_str(); }
 
-int ske_version(void) { return 10000; }
+int ske_version(void) { return 10100; }
 
 int ske_device_count(int* count) {
     if (!count) return fail(SKE_EINVAL, "count is NULL");
@@ -507,7 +745,7 @@ int ske_create_device(int dimx, int dimy, const float* d_input, int device_id, s
 
 int ske_destroy(ske_engine* e) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
-    (void)hipSetDevice(e->device);
+    (void)hipSetDevice(e->core.device);
     delete e;
     return SKE_OK;
 }
@@ -545,14 +783,15 @@ int ske_deskew(ske_engine* e, int is_max) {
 
 int ske_reduce(ske_engine* e, float* output, int is_max) {
     if (!e || !output) return fail(SKE_EINVAL, "engine or output is NULL");
-    SKE_HIP(hipSetDevice(e->device));
+    Core& c = e->core;
+    SKE_HIP(hipSetDevice(c.device));
     const size_t N = (size_t)e->N;
-    SKE_HIP(hipMemcpyAsync(e->out.p, output, N * sizeof(float), hipMemcpyHostToDevice, e->stream));
-    SKE_TRY(e->launch("reduce", [&] {
-        return launch_reduce(e->acc.p, e->present, e->out.p, e->dimx, e->dimy, is_max ? 1 : 0, e->stream);
+    SKE_HIP(hipMemcpyAsync(c.out.p, output, N * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    SKE_TRY(c.launch("reduce", [&] {
+        return launch_reduce(c.acc.p, e->present, c.out.p, e->dimx, e->dimy, is_max ? 1 : 0, c.stream);
     }));
-    SKE_HIP(hipMemcpyAsync(output, e->out.p, N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-    return e->sync();
+    SKE_HIP(hipMemcpyAsync(output, c.out.p, N * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    return c.sync();
 }
 
 int ske_get_params(ske_engine* e, ske_sector_params* p) {
@@ -586,10 +825,11 @@ int ske_get_tables(ske_engine* e, int* target, float* weight, unsigned short* fi
 int ske_get_skew(ske_engine* e, float* skewInput, float* skewOutput) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
     if (!e->have_sector) return fail(SKE_ESTATE, "no sector skewed yet");
+    Core& c = e->core;
     const size_t cells = (size_t)e->cur.H * e->cur.dim_i;
-    if (skewInput) SKE_HIP(hipMemcpyAsync(skewInput, e->S.p, cells * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-    if (skewOutput) SKE_HIP(hipMemcpyAsync(skewOutput, e->SO.p, cells * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-    return e->sync();
+    if (skewInput) SKE_HIP(hipMemcpyAsync(skewInput, e->S.p, cells * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    if (skewOutput) SKE_HIP(hipMemcpyAsync(skewOutput, e->SO.p, cells * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    return c.sync();
 }
 
 int ske_device_pointers(ske_engine* e, float** d_skewInput, float** d_skewOutput, unsigned short** d_first,
@@ -605,57 +845,40 @@ int ske_device_pointers(ske_engine* e, float** d_skewInput, float** d_skewOutput
 
 int ske_run(ske_engine* e, int kernel_id, int sector_begin, int sector_end, float povh, float* output) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
-    SKE_TRY(run_all(e, kernel_id, sector_begin, sector_end, povh));
+    Core& c = e->core;
+    SKE_TRY(c.run(kernel_id, sector_begin, sector_end, povh));
     if (output) {
-        SKE_HIP(hipMemcpyAsync(output, e->out.p, (size_t)e->N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-        return e->sync();
+        SKE_HIP(hipMemcpyAsync(output, c.out.p, (size_t)e->N * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        return c.sync();
     }
     return SKE_OK;
 }
 
 int ske_output_device(ske_engine* e, const float** d_output) {
     if (!e || !d_output) return fail(SKE_EINVAL, "engine or d_output is NULL");
-    *d_output = e->out.p;
+    *d_output = e->core.out.p;
     return SKE_OK;
 }
 
 int ske_synchronize(ske_engine* e) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
-    return e->sync();
+    return e->core.sync();
 }
 
 int ske_set_option(ske_engine* e, int option, long long value) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
-    switch (option) {
-    case SKE_OPT_SDEM_VARIANT:
-        if (value != 0 && value != 1) return fail(SKE_EINVAL, "SDEM variant must be 0 or 1");
-        e->sdem_variant = (int)value;
-        return SKE_OK;
-    case SKE_OPT_ARENA_BYTES:
-        if (value < (1LL << 20)) return fail(SKE_EINVAL, "arena budget below 1 MiB");
-        e->arena_bytes = value;
-        return SKE_OK;
-    case SKE_OPT_PROFILE:
-        e->profile = value != 0;
-        return SKE_OK;
-    default:
-        return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
-    }
+    return e->core.set_option(option, value);
 }
 
 int ske_profile_get(ske_engine* e, const char* name, double* total_ms, int* count) {
     if (!e || !name) return fail(SKE_EINVAL, "engine or name is NULL");
-    SKE_TRY(e->sync());
-    auto it = e->prof.find(name);
-    if (total_ms) *total_ms = it == e->prof.end() ? 0.0 : it->second.first;
-    if (count) *count = it == e->prof.end() ? 0 : it->second.second;
-    return SKE_OK;
+    return e->core.profile_get(name, total_ms, count);
 }
 
 int ske_profile_reset(ske_engine* e) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
-    SKE_TRY(e->sync());
-    e->prof.clear();
+    SKE_TRY(e->core.sync());
+    e->core.prof.clear();
     return SKE_OK;
 }
 
@@ -683,6 +906,89 @@ int ske_execute(int kernel_id, int dimx, int dimy, const float* input, float* ou
     ske_destroy(e);
     if (rc != SKE_OK) g_err = msg;
     return rc;
+}
+
+// ---- batches ----------------------------------------------------------------
+
+static int batch_create(int count, const int* sides, const float* const* inputs, bool on_device, int device_id,
+                        ske_batch** out) {
+    if (!out) return fail(SKE_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (count <= 0 || !sides || !inputs) return fail(SKE_EINVAL, "empty batch or NULL arrays");
+    for (int i = 0; i < count; i++) SKE_TRY(check_side(sides[i]));
+    std::unique_ptr<ske_batch> b(new ske_batch());
+    SKE_TRY(b->core.init(device_id));
+    SKE_TRY(b->core.add_images(count, sides, inputs, on_device));
+    *out = b.release();
+    return SKE_OK;
+}
+
+int ske_batch_create(int count, const int* sides, const float* const* inputs, int device_id, ske_batch** out) {
+    return batch_create(count, sides, inputs, false, device_id, out);
+}
+
+int ske_batch_create_device(int count, const int* sides, const float* const* d_inputs, int device_id,
+                            ske_batch** out) {
+    return batch_create(count, sides, d_inputs, true, device_id, out);
+}
+
+int ske_batch_destroy(ske_batch* b) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    (void)hipSetDevice(b->core.device);
+    delete b;
+    return SKE_OK;
+}
+
+int ske_batch_run(ske_batch* b, int kernel_id, int sector_begin, int sector_end, float povh) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    return b->core.run(kernel_id, sector_begin, sector_end, povh);
+}
+
+int ske_batch_output(ske_batch* b, int index, float* output) {
+    if (!b || !output) return fail(SKE_EINVAL, "batch or output is NULL");
+    Core& c = b->core;
+    if (index < 0 || index >= (int)c.imgs.size()) return fail(SKE_EINVAL, "grid index out of range");
+    const Image& im = c.imgs[index];
+    SKE_HIP(hipMemcpyAsync(output, c.out.p + im.out_off, (size_t)im.n * im.n * sizeof(float), hipMemcpyDeviceToHost,
+                           c.stream));
+    return c.sync();
+}
+
+int ske_batch_output_device(ske_batch* b, int index, const float** d_output) {
+    if (!b || !d_output) return fail(SKE_EINVAL, "batch or d_output is NULL");
+    Core& c = b->core;
+    if (index < 0 || index >= (int)c.imgs.size()) return fail(SKE_EINVAL, "grid index out of range");
+    *d_output = c.out.p + c.imgs[index].out_off;
+    return SKE_OK;
+}
+
+int ske_batch_synchronize(ske_batch* b) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    return b->core.sync();
+}
+
+int ske_batch_set_option(ske_batch* b, int option, long long value) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    return b->core.set_option(option, value);
+}
+
+int ske_batch_profile_get(ske_batch* b, const char* name, double* total_ms, int* count) {
+    if (!b || !name) return fail(SKE_EINVAL, "batch or name is NULL");
+    return b->core.profile_get(name, total_ms, count);
+}
+
+int ske_batch_profile_reset(ske_batch* b) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    SKE_TRY(b->core.sync());
+    b->core.prof.clear();
+    return SKE_OK;
+}
+
+int ske_batch_autotune(ske_batch* b, int kernel_id, float povh, int* waves, int* wgs_per_cu, int* occupancy,
+                       double* best_ms) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    if (kernel_id != SKE_KERNEL_SDEM) return fail(SKE_EUNSUPPORTED, "the autotuner covers the viewshed operator");
+    return b->core.autotune_sdem(povh, waves, wgs_per_cu, occupancy, best_ms);
 }
 
 }  // extern "C"

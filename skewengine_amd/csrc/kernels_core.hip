@@ -40,6 +40,31 @@ __global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ in0, 
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_f32(float* __restrict__ p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(double* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+static unsigned fill_blocks(size_t n) {
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3(fill_blocks(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(fill_blocks(n)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_prepare(const float* in0, float* in1, float* in2, float* in3, int dimx, int dimy,
                           hipStream_t s) {
     dim3 grid((dimx + 31) / 32, (dimy + 31) / 32);

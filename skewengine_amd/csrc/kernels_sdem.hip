@@ -20,6 +20,8 @@
 // rounds to exactly RN(a/d): two VALU operations instead of the ~10 of the
 // IEEE divide sequence.  The tests check both variants against each other
 // and against the oracle.
+#include <algorithm>
+
 #include "device_common.h"
 #include "launch.h"
 
@@ -107,9 +109,190 @@ __global__ __launch_bounds__(256) void k_op_sdem(const SecDesc* __restrict__ sec
     }
 }
 
+// ---------------------------------------------------------------------------
+// Bit-block variant.  One scan direction of one observer sees visibility
+// flags V_1..V_D (V_0 = 1: the observer's own cell).  The reference adds
+// d_c^2 - d_o^2 for every visible run (d_o, d_c] that closes before the row
+// end; since d_c^2 - d_o^2 = sum_{k=d_o}^{d_c-1} (2k+1), that total is
+//     cv = sum_{k=0..D} V_k (2k+1)  -  sum over the trailing run of (2k+1)
+//        = A - ((D+1)^2 - s^2),  s = 1 + last step with V = 0 (0 if none).
+// Steps past a lane's own scan length are recorded as visible: they extend
+// the trailing run and cancel exactly, so every lane runs the same
+// wave-uniform D (a multiple of 32).  Per step a lane only computes the
+// quotient, one compare, one max and appends the flag to a 32-bit word;
+// every 32 steps popcounts of the word update A and s:
+//     sum_{V=1} (2k+1) = (2 d0 + 65) popc(w) - 2 sum_p p w_p,
+//     sum_p p w_p = sum_i 2^i popc(w & M_i),   k = d0 + 32 - p.
+// All quantities are integers below 2^24 for rows of at most 4096 cells, so
+// the float cv of the reference (whose partial sums are exact integers
+// there) is reproduced bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kPadBits = 128;
+
+__device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) {
+    const int pc = __popc(w);
+    const int sp = __popc(w & 0xAAAAAAAAu) + 2 * __popc(w & 0xCCCCCCCCu) + 4 * __popc(w & 0xF0F0F0F0u) +
+                   8 * __popc(w & 0xFF00FF00u) + 16 * __popc(w & 0xFFFF0000u);
+    A += (2 * d0 + 65) * pc - 2 * sp;
+    if (w != 0xFFFFFFFFu) s = d0 + 33 - __builtin_ctz(~w);
+}
+
+// The visibility flag is the sign of t = M - q: rounding preserves the sign
+// of the exact difference and M == q gives +0, so sign(t) = (q > M).
+// alignbit(w, t, 31) = (w << 1) | (t >> 31) appends it in one operation.
+template <int kDir, bool kTail>
+__device__ __forceinline__ uint32_t scan_block(const float* rowj, float h, const float4* __restrict__ rtab, int d0,
+                                               int myLen, float& M) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int u = 1; u <= 32; ++u) {
+        const int d = d0 + u;
+        const float4 y = rtab[d];
+        const float a = rowj[kDir * d] - h;
+        const float q = __builtin_fmaf(a, y.x, a * y.y);
+        float t = M - q;
+        if (kTail) t = (d > myLen) ? -1.0f : t;  // past this lane's row end: record visible
+        M = fmaxf(M, q);
+        w = __builtin_amdgcn_alignbit(w, __float_as_uint(t), 31);
+    }
+    return w;
+}
+
+template <int kDir>
+__device__ __forceinline__ int scan_dir(const float* rowj, float h, const float4* __restrict__ rtab, int myLen,
+                                        int dfull, int D) {
+    float M = -2000.0f;
+    int A = 1, s = 0;
+    int d0 = 0;
+    for (; d0 + 32 <= dfull; d0 += 32) flush_block(scan_block<kDir, false>(rowj, h, rtab, d0, myLen, M), d0, A, s);
+    for (; d0 < D; d0 += 32) flush_block(scan_block<kDir, true>(rowj, h, rtab, d0, myLen, M), d0, A, s);
+    return A - ((D + 1) * (D + 1) - s * s);
+}
+
+// One skewed row: stage it, then every wave of the workgroup takes chunks
+// of 64 observers.  Ends with a barrier so the caller may restage LDS.
+template <int kWaves>
+__device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __restrict__ secs,
+                                              const float* __restrict__ in4, const int* __restrict__ target,
+                                              const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                                              const float* __restrict__ Sbuf, const float4* __restrict__ rtab,
+                                              float povh, float* __restrict__ arena, float* row) {
+    const SecDesc sd = secs[w.sec];
+    const int r = w.row;
+    const uint32_t v = fl[sd.rows + r];
+    const int f = fl_first(v);
+    const int L = fl_last(v) - f;
+    if (L <= 0) return;  // uniform; the work list holds only rows with L > 0
+    stage_row(row, sd, r, f, L, in4, target, weight, Sbuf, kPadBits);
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nchunk = (L + 63) >> 6;
+    float* so = arena + sd.so + (size_t)r * sd.dim_i + f;
+    for (int c = wave; c < nchunk; c += kWaves) {
+        const int j0 = c << 6;
+        const int jj = j0 + lane;
+        const bool valid = jj < L;
+        const bool full = j0 + 63 < L;  // all 64 observers of the chunk exist
+        const float h = row[jj] + povh;
+        // right scan: k = j+1 .. last-1
+        const int dmaxR = L - 1 - j0;
+        const int cvR = scan_dir<+1>(row + jj, h, rtab, valid ? L - 1 - jj : 0, full ? L - 1 - (j0 + 63) : 0,
+                                     (dmaxR + 31) & ~31);
+        // left scan: k = j-1 .. first
+        const int dmaxL = full ? j0 + 63 : L - 1;
+        const int cvL = scan_dir<-1>(row + jj, h, rtab, valid ? jj : 0, full ? j0 : 0, (dmaxL + 31) & ~31);
+        if (valid) so[jj] = (float)(cvR + cvL) * sd.R;
+    }
+    __syncthreads();
+}
+
+// One workgroup per row of the work list.
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __restrict__ secs,
+                                                              const RowWork* __restrict__ work,
+                                                              const float* __restrict__ in4,
+                                                              const int* __restrict__ target,
+                                                              const float* __restrict__ weight,
+                                                              const uint32_t* __restrict__ fl,
+                                                              const float* __restrict__ Sbuf,
+                                                              const float4* __restrict__ rtab, float povh,
+                                                              float* __restrict__ arena) {
+    extern __shared__ float lds[];
+    sdem_bits_row<kWaves>(work[blockIdx.x], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
+                          lds + kPadBits);
+}
+
+// Persistent: a fixed grid pulls rows off the (cost-sorted) work list with
+// an atomic counter until the list is exhausted; every workgroup exits once
+// the counter passes nwork.  The counter is cleared by the launcher.
+// kOcc > 1 asks the compiler for at least kOcc waves per SIMD (fewer VGPRs,
+// less software pipelining of the LDS loads, more waves to hide latency).
+template <int kWaves, int kOcc>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kOcc, 8))) void k_op_sdem_bits_persistent(
+    const SecDesc* __restrict__ secs, const RowWork* __restrict__ work, int nwork, int* __restrict__ counter,
+    const float* __restrict__ in4, const int* __restrict__ target, const float* __restrict__ weight,
+    const uint32_t* __restrict__ fl, const float* __restrict__ Sbuf, const float4* __restrict__ rtab, float povh,
+    float* __restrict__ arena) {
+    extern __shared__ float lds[];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item;
+        __syncthreads();  // every lane has read `item` before thread 0 overwrites it
+        if (it >= nwork) break;
+        sdem_bits_row<kWaves>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena, lds + kPadBits);
+    }
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
+                                const float4* rtab, float povh, const SdemTuning& t, int* counter, hipStream_t s) {
+    if (p.nwork == 0) return hipSuccess;
+    if (p.maxL > 4096) return launch_op_sdem(p, in4, Sbuf, arena, rtab, povh, 2, s);
+    const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPadBits);
+    if (t.persistent_wgs_per_cu > 0 && counter) {
+        int dev = 0, ncu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, counter, 1);
+        const int grid = std::min(p.nwork, ncu * t.persistent_wgs_per_cu);
+#define SKE_PERS(W, O)                                                                                           \
+    hipLaunchKernelGGL((k_op_sdem_bits_persistent<W, O>), dim3(grid), dim3(64 * W), lds, s, p.secs, p.work,       \
+                       p.nwork, counter, in4, p.target, p.weight, p.fl, Sbuf, rtab, povh, arena)
+        if (t.occupancy >= 8) {
+            if (t.waves == 2) SKE_PERS(2, 8); else if (t.waves == 8) SKE_PERS(8, 8); else SKE_PERS(4, 8);
+        } else {
+            if (t.waves == 2) SKE_PERS(2, 1); else if (t.waves == 8) SKE_PERS(8, 1); else SKE_PERS(4, 1);
+        }
+#undef SKE_PERS
+        return hipGetLastError();
+    }
+#define SKE_GRID(W)                                                                                             \
+    hipLaunchKernelGGL(k_op_sdem_bits<W>, dim3(p.nwork), dim3(64 * W), lds, s, p.secs, p.work, in4, p.target, \
+                       p.weight, p.fl, Sbuf, rtab, povh, arena)
+    if (t.waves == 2) SKE_GRID(2); else if (t.waves == 8) SKE_GRID(8); else SKE_GRID(4);
+#undef SKE_GRID
+    return hipGetLastError();
+}
+
 hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
                           const float4* rtab, float povh, int variant, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
+    if (variant == 0 && p.maxL <= 4096) {
+        const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPadBits);
+        hipLaunchKernelGGL(k_op_sdem_bits<4>, dim3(p.nwork), dim3(256), lds, s, p.secs, p.work, in4, p.target,
+                           p.weight, p.fl, Sbuf, rtab, povh, arena);
+        return hipGetLastError();
+    }
     const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPad);
     if (variant == 1)
         hipLaunchKernelGGL(k_op_sdem<1>, dim3(p.nwork), dim3(256), lds, s, p.secs, p.work, in4, p.target,

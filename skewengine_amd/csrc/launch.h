@@ -21,6 +21,17 @@ struct DevPlan {
 
 enum DeskewSource { kDeskewDense = 0, kDeskewArena = 1, kDeskewRadon = 2 };
 
+// Launch shape of the viewshed operator, chosen by the autotuner.
+struct SdemTuning {
+    int waves = 4;                 // waves per workgroup (observer chunks of one row in parallel): 2, 4 or 8
+    int persistent_wgs_per_cu = 0; // 0: one workgroup per row; >0: persistent grid of this many per CU
+    int occupancy = 0;             // persistent grid: 8 = compile for 8 waves per SIMD (fewer VGPRs)
+};
+
+// Graph-safe clears (kernels, not hipMemsetAsync).
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_f64(double* p, size_t n, hipStream_t s);
+
 // prepare(): the three mirrored/transposed copies (src/skewEngine.cuh:962-987).
 hipError_t launch_prepare(const float* in0, float* in1, float* in2, float* in3, int dimx, int dimy,
                           hipStream_t s);
@@ -36,6 +47,11 @@ hipError_t launch_op_unit(const DevPlan& p, const float* in4, const float* Sbuf,
                           hipStream_t s);
 hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
                           const float4* rtab, float povh, int variant, hipStream_t s);
+// Bit-block viewshed with an explicit launch shape; `counter` (one int of
+// device memory) feeds the persistent grid.
+hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
+                                const float4* rtab, float povh, const SdemTuning& t, int* counter,
+                                hipStream_t s);
 // tab: cos/sin(2 pi t/64) interleaved (128 doubles) then cos(2 pi t/32) (32).
 hipError_t launch_op_blur(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
                           const double* tab, hipStream_t s);

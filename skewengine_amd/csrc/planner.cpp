@@ -62,15 +62,41 @@ void plan_sector(int dimx, int dimy, int angle, SectorPlan& sp) {
 void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_base, long long q_base, int order,
                 ImagePlan& ip) {
     ip = ImagePlan();
+    ip.so_total = so_base;
+    ip.q_total = q_base;
+    plan_append(dimx, dimy, s0, s1, img, ip, ip.range);
+    ip.so_total -= so_base;
+    ip.q_total -= q_base;
+    plan_finish(ip, order);
+}
+
+void plan_finish(ImagePlan& ip, int order) {
+    // ip.work holds rows in natural order; sort by descending cost so that a
+    // dynamic scheduler hands out the long rows first.
+    std::vector<std::pair<double, RowWork>> rows;
+    rows.reserve(ip.work.size());
+    for (const RowWork& w : ip.work) {
+        const SecDesc& sd = ip.secs[w.sec];
+        const uint32_t v = ip.fl[sd.rows + w.row];
+        const double L = (double)((int)(v >> 16) - (int)(v & 0xffffu));
+        rows.push_back({order == kOrderCostDesc ? L * L : L, w});
+    }
+    std::stable_sort(rows.begin(), rows.end(),
+                     [](const std::pair<double, RowWork>& x, const std::pair<double, RowWork>& y) {
+                         return x.first > y.first;
+                     });
+    for (size_t i = 0; i < rows.size(); i++) ip.work[i] = rows[i].second;
+}
+
+void plan_append(int dimx, int dimy, int s0, int s1, long long img, ImagePlan& ip, int range8[8]) {
     ip.dimx = dimx;
     ip.dimy = dimy;
     ip.s0 = s0;
     ip.s1 = s1;
-    for (int t = 0; t < 4; t++) ip.range[2 * t] = ip.range[2 * t + 1] = 0;
+    for (int t = 0; t < 8; t++) range8[t] = 0;
     int seen[4] = {0, 0, 0, 0};
     SectorPlan sp;
-    long long so = so_base, q = q_base;
-    std::vector<std::pair<double, RowWork>> rows;
+    long long so = ip.so_total, q = ip.q_total;
     for (int a = s0; a < s1; a++) {
         plan_sector(dimx, dimy, a, sp);
         SecDesc d{};
@@ -87,6 +113,7 @@ void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_
         // The deskew's zero-weight tap of a newAngle == 0 sector reads row H,
         // which holds what the previously processed sector left there.
         if (sp.newAngle == 0 && a > s0) d.stale = (int)ip.secs.size() - 1;
+        d.pad0 = 0;
         d.R = (float)(sp.scale * sp.scale);  // (float)pow(scale,2), src/kernelSDEM.h:171
         d.rscale = (float)(1 / sp.scale);
         d.img = img;
@@ -101,8 +128,7 @@ void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_
             ip.fl.push_back((uint32_t)f | ((uint32_t)l << 16));
             const int L = l - f;
             if (L > 0) {
-                const double c = (order == kOrderCostDesc) ? (double)L * L : (double)L;
-                rows.push_back({c, RowWork{si, r}});
+                ip.work.push_back(RowWork{si, r});
                 ip.maxL = std::max(ip.maxL, L);
                 ip.cells += L;
                 ip.steps += (double)L * (L - 1);
@@ -113,19 +139,13 @@ void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_
         q += sp.H;
         if (!seen[sp.type]) {
             seen[sp.type] = 1;
-            ip.range[2 * sp.type] = si;
+            range8[2 * sp.type] = si;
         }
-        ip.range[2 * sp.type + 1] = si + 1;
+        range8[2 * sp.type + 1] = si + 1;
         ip.secs.push_back(d);
     }
-    std::stable_sort(rows.begin(), rows.end(),
-                     [](const std::pair<double, RowWork>& x, const std::pair<double, RowWork>& y) {
-                         return x.first > y.first;
-                     });
-    ip.work.reserve(rows.size());
-    for (auto& pr : rows) ip.work.push_back(pr.second);
-    ip.so_total = so - so_base;
-    ip.q_total = q - q_base;
+    ip.so_total = so;
+    ip.q_total = q;
 }
 
 void recip_table(int n, std::vector<float>& tab4) {

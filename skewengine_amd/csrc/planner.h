@@ -52,6 +52,13 @@ enum WorkOrder { kOrderNatural = 0, kOrderCostDesc = 1 };
 void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_base, long long q_base,
                 int order, ImagePlan& ip);
 
+// Append sectors [s0, s1) of one more image to `ip` (arena and projection
+// offsets continue from ip.so_total / ip.q_total); range8 receives the
+// image's per-type [begin, end) indices into ip.secs.  Call plan_finish once
+// after the last append to order the row work list.
+void plan_append(int dimx, int dimy, int s0, int s1, long long img, ImagePlan& ip, int range8[8]);
+void plan_finish(ImagePlan& ip, int order);
+
 // {RN(1/d), RN(1/d - RN(1/d)), d, d*d} for d = 0..n-1.
 void recip_table(int n, std::vector<float>& tab4);
 

@@ -127,6 +127,84 @@ class SkewEngine:
         check(lib().ske_profile_reset(self._h))
 
 
+class SkewBatch:
+    """Many square grids of uneven sizes through one all-sector run.
+
+    No reference counterpart: each grid's result equals SkewEngine(grid).run(...).
+    """
+
+    def __init__(self, grids, device_id=0):
+        self._grids = [_f32(g) for g in grids]
+        for g in self._grids:
+            if g.ndim != 2 or g.shape[0] != g.shape[1]:
+                raise ValueError("batch grids must be square 2-D arrays")
+        self.sides = [g.shape[0] for g in self._grids]
+        n = len(self._grids)
+        sides = (ctypes.c_int * n)(*self.sides)
+        ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data for g in self._grids])
+        self._h = ctypes.c_void_p()
+        check(lib().ske_batch_create(n, sides, ptrs, device_id, ctypes.byref(self._h)))
+        self._grids = None  # copied to the device
+
+    def close(self):
+        if self._h:
+            check(lib().ske_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.sides)
+
+    def run(self, kernel_id, sector_begin=0, sector_end=180, povh=POVH_DEFAULT):
+        """Enqueue the run (asynchronous)."""
+        check(lib().ske_batch_run(self._h, int(kernel_id), int(sector_begin), int(sector_end), float(povh)))
+
+    def output(self, index):
+        n = self.sides[index]
+        out = np.empty((n, n), np.float32)
+        check(lib().ske_batch_output(self._h, int(index), out.ctypes.data))
+        return out
+
+    def outputs(self):
+        return [self.output(i) for i in range(len(self))]
+
+    def synchronize(self):
+        check(lib().ske_batch_synchronize(self._h))
+
+    def set_option(self, option, value):
+        check(lib().ske_batch_set_option(self._h, int(option), int(value)))
+
+    def profile(self, name):
+        ms = ctypes.c_double(0)
+        cnt = ctypes.c_int(0)
+        check(lib().ske_batch_profile_get(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def profile_reset(self):
+        check(lib().ske_batch_profile_reset(self._h))
+
+    def autotune(self, kernel_id=KERNEL_SDEM, povh=POVH_DEFAULT):
+        """Pick the viewshed launch shape: returns (waves, persistent_wgs_per_cu, occupancy, best_ms)."""
+        w = ctypes.c_int(0)
+        p = ctypes.c_int(0)
+        o = ctypes.c_int(0)
+        ms = ctypes.c_double(0)
+        check(lib().ske_batch_autotune(self._h, int(kernel_id), float(povh), ctypes.byref(w), ctypes.byref(p),
+                                       ctypes.byref(o), ctypes.byref(ms)))
+        return w.value, p.value, o.value, ms.value
+
+
 def execute(kernel_id, grid, povh=POVH_DEFAULT, sector_begin=0, sector_end=180):
     """The whole reference driver (allocate, prepare, sector farm, reduce) on one GPU."""
     grid = _f32(grid)

@@ -125,9 +125,22 @@ def test_viewshed_c1_sizes_against_multithread_oracle():
         fast = e.run(ske.KERNEL_SDEM)
         e.set_option(1, 1)
         ieee = e.run(ske.KERNEL_SDEM)
+        e.set_option(1, 2)
+        fma = e.run(ske.KERNEL_SDEM)
     assert np.array_equal(fast, ieee)
+    assert np.array_equal(fast, fma)
     ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=0)
     assert _rel(fast, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_viewshed_variants_bit_exact_vs_oracle(variant):
+    grid = synth.dem(96, 0.9, 21, 3000.0)
+    with ske.SkewEngine(grid) as e:
+        e.set_option(1, variant)
+        out = e.run(ske.KERNEL_SDEM)
+    ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=1)
+    assert np.array_equal(out, ref)
 
 
 def test_viewshed_bit_exact_sectors_at_500():
