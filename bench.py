@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -120,6 +122,81 @@ def bench_viewshed_batch(args, ske, synth):
     return result, steps_total, pts_sectors
 
 
+def _timed(fn, sync, reps):
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def extra_c1_viewshed(ske, synth):
+    """Config 1: one 500^2 DEM (H=0.8, 0-1500 m), 180 sectors."""
+    g = synth.dem(500, 0.8, 1, 1500.0)
+    _, steps = ske.work(500, 500)
+    with ske.SkewEngine(g) as e:
+        t = _timed(lambda: e.run(ske.KERNEL_SDEM, copy_out=False), e.synchronize, 20)
+    return {"workload": "500x500 fBm DEM, 180 sectors", "ms": t * 1e3,
+            "Mpts_sectors_per_s": 500 * 500 * 180 / 1e6 / t, "ray_steps_per_s": steps / t}
+
+
+def extra_c2_radon(ske, synth):
+    """Config 2: 512^2 ellipse phantom, 180 angles -- the reference-semantics
+    skew-space Radon/FBP operator, and the standard 725-bin projector + Ram-Lak
+    FBP, in slices/s."""
+    from skewengine_amd import _lib
+    from skewengine_amd.fbp import FBP
+    img = synth.ellipse_phantom(512, seed=2)
+    res = {}
+    with ske.SkewEngine(img) as e:
+        t = _timed(lambda: e.run(ske.KERNEL_RADON, copy_out=False), e.synchronize, 20)
+    res["skew_radon_fbp"] = {"workload": "512x512 phantom, 180 skew sectors (kernelRADN semantics)",
+                             "ms_per_slice": t * 1e3, "slices_per_s": 1.0 / t}
+    batch = 16
+    imgs = np.stack([synth.ellipse_phantom(512, seed=100 + i) for i in range(batch)])
+    with FBP(512, 180, 725, max_batch=batch) as f:
+        f.upload(0, imgs)
+        f.set_profile(True)
+
+        def step():
+            f.project(batch)
+            f.reconstruct(batch)
+        t = _timed(step, f.synchronize, 5)
+        f.profile_reset()
+        step()
+        f.synchronize()
+        k = {name: f.profile(name)[0] / batch for name in ("fbp_project", "fbp_filter", "fbp_backproject")}
+    res["standard_radon_plus_fbp"] = {
+        "workload": "512x512 phantoms, 180 angles x 725 bins, project + Ram-Lak filter + backproject, batch %d" % batch,
+        "ms_per_slice": t * 1e3 / batch, "slices_per_s": batch / t, "kernel_ms_per_slice": k}
+    del _lib
+    return res
+
+
+def extra_c3_cepstrum(ske, synth, count):
+    """Config 3: batched cepstrum blur estimation over `count` 1024^2 images."""
+    from skewengine_amd import _lib
+    imgs = [synth.blurred_texture(1024, 0.6, seed=i)[0] for i in range(count)]
+    cells, _ = ske.work(1024, 1024)
+    with ske.SkewBatch(imgs) as b:
+        del imgs
+        b.set_option(_lib.OPT_PROFILE, 1)
+        b.run(ske.KERNEL_CEPSTRUM)
+        b.synchronize()
+        b.profile_reset()
+        t0 = time.perf_counter()
+        b.run(ske.KERNEL_CEPSTRUM)
+        b.synchronize()
+        t = time.perf_counter() - t0
+        k_ms, k_cnt = b.profile("op_blur")
+    total = cells * count
+    return {"workload": "%d x 1024^2 fBm (H=0.6) + motion blur + noise, 180 sectors, batched" % count,
+            "s": t, "cell_sectors_per_s": total / t, "op_blur_ms": k_ms, "op_blur_launches": k_cnt,
+            "op_blur_TFLOPs_fp64_at_2848_per_cell": 2848 * total / (k_ms / 1e3) / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +204,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations 1-3")
+    ap.add_argument("--cepstrum-images", type=int, default=64)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -145,6 +224,10 @@ def main():
         "data": "synthetic (seeded fBm spectral synthesis)", "config": res["config"],
         "roofline": res["roofline"],
     }
+    if not args.no_extra:
+        line["extra"] = {"c1_viewshed_500": extra_c1_viewshed(ske, synth),
+                         "c2_radon_fbp_512": extra_c2_radon(ske, synth),
+                         "c3_cepstrum_batch": extra_c3_cepstrum(ske, synth, args.cepstrum_images)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_viewshed(steps_total, pts_sectors, args.cpu_seconds)
         line["speedup_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]

@@ -168,6 +168,38 @@ int ske_batch_profile_reset(ske_batch *b);
 int ske_batch_autotune(ske_batch *b, int kernel_id, float povh, int *waves, int *wgs_per_cu, int *occupancy,
                        double *best_ms);
 
+/* ---- Standard parallel-beam Radon / FBP.  No reference counterpart: the
+ * reference's Radon operator (src/kernelRADN.h:212-259) works in its own
+ * skewed geometry; this is the 725-bin geometry of the benchmark
+ * configurations.  Image n x n on [-1,1]^2 (row 0 at y=+1), bins
+ * s_k = (k-(n_det-1)/2)*2/n, ray samples on the same pitch with bilinear
+ * interpolation, Ram-Lak filter as a (2*n_det-1)-tap convolution,
+ * back-projection (pi/n_angles) * sum_a q(a, x cos + y sin), linear in s. */
+typedef struct ske_fbp ske_fbp;
+#define SKE_FBP_IMAGE 0    /* batch x n x n */
+#define SKE_FBP_SINO 1     /* batch x n_angles x n_det */
+#define SKE_FBP_FILTERED 2 /* batch x n_angles x n_det */
+#define SKE_FBP_RECON 3    /* batch x n x n */
+/* angles: n_angles radians, or NULL for a*pi/n_angles, a = 0..n_angles-1. */
+int ske_fbp_create(int n, int n_angles, int n_det, const float *angles, int max_batch, int device_id,
+                   ske_fbp **out);
+int ske_fbp_destroy(ske_fbp *f);
+int ske_fbp_upload(ske_fbp *f, int buffer, int batch, const float *host);
+int ske_fbp_download(ske_fbp *f, int buffer, int batch, float *host);
+int ske_fbp_buffer(ske_fbp *f, int buffer, float **d_ptr);
+int ske_fbp_project(ske_fbp *f, int batch);     /* IMAGE -> SINO */
+int ske_fbp_filter(ske_fbp *f, int batch);      /* SINO -> FILTERED with the current taps */
+int ske_fbp_backproject(ske_fbp *f, int batch); /* FILTERED -> RECON */
+int ske_fbp_reconstruct(ske_fbp *f, int batch); /* filter + backproject */
+/* 2*n_det-1 filter taps h(m), m = -(n_det-1)..n_det-1 (the bin pitch folded
+ * in); NULL restores Ram-Lak. */
+int ske_fbp_set_taps(ske_fbp *f, const float *taps);
+int ske_fbp_get_taps(ske_fbp *f, float *taps);
+int ske_fbp_synchronize(ske_fbp *f);
+int ske_fbp_set_option(ske_fbp *f, int option, long long value); /* SKE_OPT_PROFILE */
+int ske_fbp_profile_get(ske_fbp *f, const char *name, double *total_ms, int *count);
+int ske_fbp_profile_reset(ske_fbp *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,9 @@ EXPORTS = [
     "ske_batch_create", "ske_batch_create_device", "ske_batch_destroy", "ske_batch_run", "ske_batch_output",
     "ske_batch_output_device", "ske_batch_synchronize", "ske_batch_set_option", "ske_batch_profile_get",
     "ske_batch_profile_reset", "ske_batch_autotune",
+    "ske_fbp_create", "ske_fbp_destroy", "ske_fbp_upload", "ske_fbp_download", "ske_fbp_buffer", "ske_fbp_project",
+    "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct", "ske_fbp_set_taps", "ske_fbp_get_taps",
+    "ske_fbp_synchronize", "ske_fbp_set_option", "ske_fbp_profile_get", "ske_fbp_profile_reset",
 ]
 
 
@@ -97,6 +100,19 @@ def lib():
     L.ske_batch_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_batch_profile_reset.argtypes = [vp]
     L.ske_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_float, i32p, i32p, i32p, f64p]
+    L.ske_fbp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, P(vp)]
+    L.ske_fbp_destroy.argtypes = [vp]
+    L.ske_fbp_upload.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_fbp_download.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_fbp_buffer.argtypes = [vp, ctypes.c_int, P(vp)]
+    for name in ("ske_fbp_project", "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct"):
+        getattr(L, name).argtypes = [vp, ctypes.c_int]
+    L.ske_fbp_set_taps.argtypes = [vp, vp]
+    L.ske_fbp_get_taps.argtypes = [vp, vp]
+    L.ske_fbp_synchronize.argtypes = [vp]
+    L.ske_fbp_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.ske_fbp_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
+    L.ske_fbp_profile_reset.argtypes = [vp]
     _lib = L
     return L
 

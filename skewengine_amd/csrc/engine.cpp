@@ -31,6 +31,7 @@
 #include <tuple>
 #include <vector>
 
+#include "host_util.h"
 #include "launch.h"
 #include "planner.h"
 #include "skewengine.h"
@@ -38,59 +39,6 @@
 using namespace ske;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define SKE_HIP(x)                                                                                   \
-    do {                                                                                             \
-        hipError_t e_ = (x);                                                                         \
-        if (e_ != hipSuccess) return fail(SKE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define SKE_TRY(x)                     \
-    do {                               \
-        int rc_ = (x);                 \
-        if (rc_ != SKE_OK) return rc_; \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    // Grow to hold at least `count` elements (contents not preserved).
-    int ensure(size_t count) {
-        if (count <= n && p) return SKE_OK;
-        release();
-        size_t c = std::max<size_t>(count, 1);
-        hipError_t e = hipMalloc(&p, c * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(SKE_ENOMEM, std::string("hipMalloc(") + std::to_string(c * sizeof(T)) +
-                                        " bytes): " + hipGetErrorString(e));
-        }
-        n = c;
-        return SKE_OK;
-    }
-    int upload(const T* h, size_t count, hipStream_t s) {
-        SKE_TRY(ensure(count));
-        if (count) SKE_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
-        return SKE_OK;
-    }
-};
 
 struct DevPlanBuf {
     ImagePlan ip;
@@ -725,7 +673,7 @@ int sector_deskew(ske_engine* e, int is_max) {
 
 extern "C" {
 
-const char* ske_last_error(void) { return g_err.c_str(); }
+const char* ske_last_error(void) { return last_error().c_str(); }
 
 int ske_version(void) { return 10100; }
 
@@ -902,9 +850,9 @@ int ske_execute(int kernel_id, int dimx, int dimy, const float* input, float* ou
     ske_engine* e = nullptr;
     SKE_TRY(ske_create(dimx, dimy, input, 0, &e));
     int rc = ske_run(e, kernel_id, sector_begin, sector_end, povh, output);
-    std::string msg = g_err;
+    std::string msg = last_error();
     ske_destroy(e);
-    if (rc != SKE_OK) g_err = msg;
+    if (rc != SKE_OK) last_error() = msg;
     return rc;
 }
 

@@ -1,0 +1,26 @@
+// fbp.h -- standard parallel-beam geometry shared by the FBP host object and
+// its kernels (kernels_fbp.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ske {
+
+struct FbpGeom {
+    int n;        // image side
+    int nang;     // projection angles
+    int ndet;     // detector bins
+    int nt;       // ray samples per bin (= ndet)
+    float delta;  // pixel pitch and bin spacing, 2/n
+    float det_c;  // (ndet-1)/2
+    float tmax;   // (nt-1)/2 * delta
+    float pad;
+};
+
+hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const FbpGeom& g, int batch,
+                              hipStream_t s);
+hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, const FbpGeom& g, int batch,
+                             hipStream_t s);
+hipError_t launch_fbp_backproject(const float* filt, float* img, const float2* cs, const FbpGeom& g, int batch,
+                                  float scale, hipStream_t s);
+
+}  // namespace ske

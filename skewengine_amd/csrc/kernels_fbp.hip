@@ -1,0 +1,179 @@
+// kernels_fbp.hip -- standard parallel-beam geometry (no reference code; the
+// counterpart of the reference's skew-space Radon, src/kernelRADN.h:212-259,
+// in the 725-bin geometry of the benchmark configurations):
+//
+//   image   n x n on [-1,1]^2, pixel pitch delta = 2/n, row 0 at y = +1;
+//   angles  theta_a (host table of cos/sin);
+//   bins    s_k = (k - (ndet-1)/2) * delta.
+//
+//   project     p(a,k) = delta * sum_t f(s_k cos - t sin, s_k sin + t cos),
+//               t on a delta grid over the chord of the ray through the
+//               image square, f bilinear (zero outside);
+//   filter      q(a,k) = sum_k' p(a,k') h(k-k'), h = delta * Ram-Lak
+//               (h0 = 1/(4 delta^2), odd m: -1/(m^2 pi^2 delta^2), even: 0) or
+//               learned taps;
+//   backproject f(x,y) = (pi/nang) sum_a q(a, s(x,y)), linear in s.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "fbp.h"
+
+namespace ske {
+
+__device__ __forceinline__ float bilinear(const float* __restrict__ img, int n, float u, float v) {
+    // u: column coordinate, v: row coordinate, pixel centres at integers
+    const float fu = floorf(u), fv = floorf(v);
+    const int j0 = (int)fu, i0 = (int)fv;
+    const float wu = u - fu, wv = v - fv;
+    float a = 0.0f, b = 0.0f, c = 0.0f, d = 0.0f;
+    const bool j0in = j0 >= 0 && j0 < n, j1in = j0 + 1 >= 0 && j0 + 1 < n;
+    if (i0 >= 0 && i0 < n) {
+        const float* r = img + (size_t)i0 * n;
+        if (j0in) a = r[j0];
+        if (j1in) b = r[j0 + 1];
+    }
+    if (i0 + 1 >= 0 && i0 + 1 < n) {
+        const float* r = img + (size_t)(i0 + 1) * n;
+        if (j0in) c = r[j0];
+        if (j1in) d = r[j0 + 1];
+    }
+    const float top = a + wu * (b - a);
+    const float bot = c + wu * (d - c);
+    return top + wv * (bot - top);
+}
+
+// One thread per (bin, angle, slice).
+__global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ img, float* __restrict__ sino,
+                                                     const float2* __restrict__ cs, FbpGeom g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    const int b = blockIdx.z;
+    if (k >= g.ndet) return;
+    const float2 t2 = cs[a];
+    const float c = t2.x, sn = t2.y;
+    const float s = ((float)k - g.det_c) * g.delta;
+    // The bilinear image is zero farther than 1 + delta/2 from the centre
+    // along either axis, so samples outside the chord of the line
+    // x c + y sn = s through the square [-B, B]^2, B = 1 + delta, add exact
+    // zeros and are skipped (x = s c - t sn, y = s sn + t c).
+    const float B = 1.0f + g.delta;
+    float tlo = -g.tmax, thi = g.tmax;
+    if (fabsf(sn) > 1e-7f) {
+        const float t1 = (s * c - B) / sn, t2v = (s * c + B) / sn;
+        tlo = fmaxf(tlo, fminf(t1, t2v));
+        thi = fminf(thi, fmaxf(t1, t2v));
+    } else if (fabsf(s * c) > B) {
+        thi = tlo - 1.0f;
+    }
+    if (fabsf(c) > 1e-7f) {
+        const float t1 = (-B - s * sn) / c, t2v = (B - s * sn) / c;
+        tlo = fmaxf(tlo, fminf(t1, t2v));
+        thi = fminf(thi, fmaxf(t1, t2v));
+    } else if (fabsf(s * sn) > B) {
+        thi = tlo - 1.0f;
+    }
+    const float* im = img + (size_t)b * g.n * g.n;
+    float acc = 0.0f;
+    if (thi >= tlo) {
+        // samples t_i = (i - (nt-1)/2) * delta, i in [i0, i1] covering [tlo, thi]
+        const float half = 0.5f * (float)(g.nt - 1);
+        int i0 = (int)ceilf(tlo / g.delta + half) - 1;
+        int i1 = (int)floorf(thi / g.delta + half) + 1;
+        i0 = max(i0, 0);
+        i1 = min(i1, g.nt - 1);
+        const float inv = 1.0f / g.delta;
+        for (int i = i0; i <= i1; ++i) {
+            const float t = ((float)i - half) * g.delta;
+            const float x = s * c - t * sn;
+            const float y = s * sn + t * c;
+            acc += bilinear(im, g.n, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
+        }
+    }
+    sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
+}
+
+// One workgroup per sinogram row: the row and the 2*ndet-1 filter taps in LDS.
+__global__ __launch_bounds__(256) void k_fbp_filter(const float* __restrict__ sino, float* __restrict__ filt,
+                                                    const float* __restrict__ taps, int ndet) {
+    extern __shared__ float sm[];
+    float* p = sm;
+    float* h = sm + ndet;
+    const size_t row = blockIdx.x;
+    const float* src = sino + row * ndet;
+    for (int i = threadIdx.x; i < ndet; i += blockDim.x) p[i] = src[i];
+    for (int i = threadIdx.x; i < 2 * ndet - 1; i += blockDim.x) h[i] = taps[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < ndet; k += blockDim.x) {
+        float acc = 0.0f;
+        const float* hk = h + k + ndet - 1;  // h(k - k') at hk[-k']
+        for (int kp = 0; kp < ndet; ++kp) acc = __builtin_fmaf(p[kp], hk[-kp], acc);
+        filt[row * ndet + k] = acc;
+    }
+}
+
+// One thread per pixel; angles in chunks staged in LDS.
+// chunk: angles staged per pass (chunk * ndet floats <= 64 KB).
+__global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict__ filt, float* __restrict__ img,
+                                                         const float2* __restrict__ cs, FbpGeom g, float scale,
+                                                         int chunk) {
+    extern __shared__ float q[];  // chunk * ndet
+    const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int b = blockIdx.z;
+    const bool in = i < g.n && j < g.n;
+    const float x = ((float)j + 0.5f) * g.delta - 1.0f;
+    const float y = 1.0f - ((float)i + 0.5f) * g.delta;
+    const float inv = 1.0f / g.delta;
+    const float* fb = filt + (size_t)b * g.nang * g.ndet;
+    float acc = 0.0f;
+    for (int a0 = 0; a0 < g.nang; a0 += chunk) {
+        const int na = min(chunk, g.nang - a0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < na * g.ndet; t += blockDim.x) q[t] = fb[(size_t)a0 * g.ndet + t];
+        __syncthreads();
+        if (in) {
+            for (int aa = 0; aa < na; ++aa) {
+                const float2 t2 = cs[a0 + aa];
+                const float s = x * t2.x + y * t2.y;
+                const float kf = s * inv + g.det_c;
+                const float fk = floorf(kf);
+                const int k0 = (int)fk;
+                const float w = kf - fk;
+                const float* qa = q + aa * g.ndet;
+                const float v0 = (k0 >= 0 && k0 < g.ndet) ? qa[k0] : 0.0f;
+                const float v1 = (k0 + 1 >= 0 && k0 + 1 < g.ndet) ? qa[k0 + 1] : 0.0f;
+                acc += v0 + w * (v1 - v0);
+            }
+        }
+    }
+    if (in) img[(size_t)b * g.n * g.n + (size_t)i * g.n + j] = acc * scale;
+}
+
+hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const FbpGeom& g, int batch,
+                              hipStream_t s) {
+    dim3 grid((g.ndet + 255) / 256, g.nang, batch);
+    hipLaunchKernelGGL(k_fbp_project, grid, dim3(256), 0, s, img, sino, cs, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, const FbpGeom& g, int batch,
+                             hipStream_t s) {
+    const size_t lds = sizeof(float) * (size_t)(3 * g.ndet - 1);
+    hipLaunchKernelGGL(k_fbp_filter, dim3((unsigned)((size_t)batch * g.nang)), dim3(256), lds, s, sino, filt, taps,
+                       g.ndet);
+    return hipGetLastError();
+}
+
+hipError_t launch_fbp_backproject(const float* filt, float* img, const float2* cs, const FbpGeom& g, int batch,
+                                  float scale, hipStream_t s) {
+    dim3 grid((g.n + 15) / 16, (g.n + 15) / 16, batch);
+    const int chunk = std::max(1, std::min(16, 16384 / g.ndet));
+    const size_t lds = sizeof(float) * (size_t)chunk * g.ndet;
+    hipLaunchKernelGGL(k_fbp_backproject, grid, dim3(256), lds, s, filt, img, cs, g, scale, chunk);
+    return hipGetLastError();
+}
+
+}  // namespace ske
