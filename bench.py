@@ -146,7 +146,6 @@ def extra_c2_radon(ske, synth):
     """Config 2: 512^2 ellipse phantom, 180 angles -- the reference-semantics
     skew-space Radon/FBP operator, and the standard 725-bin projector + Ram-Lak
     FBP, in slices/s."""
-    from skewengine_amd import _lib
     from skewengine_amd.fbp import FBP
     img = synth.ellipse_phantom(512, seed=2)
     res = {}
@@ -171,7 +170,6 @@ def extra_c2_radon(ske, synth):
     res["standard_radon_plus_fbp"] = {
         "workload": "512x512 phantoms, 180 angles x 725 bins, project + Ram-Lak filter + backproject, batch %d" % batch,
         "ms_per_slice": t * 1e3 / batch, "slices_per_s": batch / t, "kernel_ms_per_slice": k}
-    del _lib
     return res
 
 

@@ -200,6 +200,52 @@ int ske_fbp_set_option(ske_fbp *f, int option, long long value); /* SKE_OPT_PROF
 int ske_fbp_profile_get(ske_fbp *f, const char *name, double *total_ms, int *count);
 int ske_fbp_profile_reset(ske_fbp *f);
 
+/* ---- Learned sparse-view FBP.  No reference counterpart: it extends the
+ * Radon/FBP operator (src/kernelRADN.h:212-259) with a learnable filter and
+ * a residual CNN, in the geometry of an ske_fbp whose stream, timer and
+ * buffers it shares (SINO in, RECON out, IMAGE = training target):
+ *   r0  = BP(h * p), h[m] = taps[|m|] (n_det symmetric taps; the DCT of the
+ *         taps is the filter's per-frequency response)
+ *   out = r0 + conv(relu(conv(relu(conv(relu(conv(r0))))))) with 3x3 convs
+ *         1->32->32->32->1, zero padding, bf16 weights and activations,
+ *         fp32 accumulation.
+ * Parameters (fp32, SKE_LEARNED_TAPS + n_det of them): W1[32][9] b1[32]
+ * W2[32][9][32] b2[32] W3[32][9][32] b3[32] W4[9][32] b4 taps[n_det]; tap
+ * t = 3*ky + kx reads pixel (i+ky-1, j+kx-1).  The image side must be a
+ * multiple of 32. */
+typedef struct ske_learned ske_learned;
+#define SKE_LEARNED_TAPS 19105 /* offset of the filter taps in the parameters */
+#define SKE_LEARNED_R0 0       /* fp32 batch x n x n: learned-filter FBP */
+#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 32: hidden activations */
+#define SKE_LEARNED_A2 2
+#define SKE_LEARNED_A3 3
+#define SKE_LEARNED_DR0 4      /* fp32 batch x n x n: d loss / d r0 (after a gradient) */
+#define SKE_LEARNED_FILTERED_GRAD 5 /* fp32 batch x n_angles x n_det: d loss / d filtered sinogram */
+int ske_learned_create(ske_fbp *f, ske_learned **out);
+int ske_learned_destroy(ske_learned *l);
+int ske_learned_param_count(ske_learned *l, int *count);
+/* He-normal hidden weights from seed, zero biases, zero output layer and
+ * Ram-Lak taps: the output equals plain FBP until trained. */
+int ske_learned_init(ske_learned *l, unsigned long long seed);
+int ske_learned_set_params(ske_learned *l, const float *params); /* also resets the optimiser */
+int ske_learned_get_params(ske_learned *l, float *params);
+int ske_learned_reconstruct(ske_learned *l, int batch); /* SINO -> RECON */
+/* loss = mean((out - IMAGE)^2) over the batch and its gradient with respect
+ * to every parameter (grad: param_count floats, or NULL). */
+int ske_learned_gradient(ske_learned *l, int batch, double *loss, float *grad);
+/* One Adam step (0.9, 0.999, 1e-8): lr for the CNN, lr_taps for the taps. */
+int ske_learned_train_step(ske_learned *l, int batch, float lr, float lr_taps, double *loss);
+int ske_learned_download(ske_learned *l, int buffer, int batch, void *host);
+/* Random ellipses (x0, y0, a, b, phi, rho): centres uniform in the unit disk,
+ * semi-axes U[0.05,0.4], rotation U[0,pi), intensity U[-0.5,1.0]. */
+int ske_random_ellipses(unsigned long long seed, int batch, int n_ellipses, float *out);
+/* batch x n_ellipses ellipses summed at pixel centres into IMAGE. */
+int ske_learned_render(ske_learned *l, int batch, int n_ellipses, const float *ellipses);
+/* Training pairs: random phantoms (seed) into IMAGE, projected into SINO, then
+ * Poisson transmission noise: counts ~ Poisson(i0 exp(-mu p)),
+ * p = -ln(max(counts, 1) / i0) / mu (i0 <= 0: no noise). */
+int ske_learned_make_batch(ske_learned *l, int batch, int n_ellipses, unsigned long long seed, float i0, float mu);
+
 #ifdef __cplusplus
 }
 #endif

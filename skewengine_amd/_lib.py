@@ -13,6 +13,8 @@ SKE_OK = 0
 KERNEL_SDEM, KERNEL_IDENTITYDEM, KERNEL_RADON, KERNEL_IDENTITYIMG, KERNEL_CEPSTRUM = 0, 1, 2, 3, 4
 OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE, OPT_GRAPH, OPT_SDEM_WAVES, OPT_SDEM_PERSISTENT = 1, 2, 3, 4, 5, 6
 OPT_SDEM_OCCUPANCY = 7
+LEARNED_TAPS = 19105
+LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180
 
 # Every symbol include/skewengine.h declares.
@@ -28,6 +30,10 @@ EXPORTS = [
     "ske_fbp_create", "ske_fbp_destroy", "ske_fbp_upload", "ske_fbp_download", "ske_fbp_buffer", "ske_fbp_project",
     "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct", "ske_fbp_set_taps", "ske_fbp_get_taps",
     "ske_fbp_synchronize", "ske_fbp_set_option", "ske_fbp_profile_get", "ske_fbp_profile_reset",
+    "ske_learned_create", "ske_learned_destroy", "ske_learned_param_count", "ske_learned_init",
+    "ske_learned_set_params", "ske_learned_get_params", "ske_learned_reconstruct", "ske_learned_gradient",
+    "ske_learned_train_step", "ske_learned_download", "ske_random_ellipses", "ske_learned_render",
+    "ske_learned_make_batch",
 ]
 
 
@@ -113,6 +119,20 @@ def lib():
     L.ske_fbp_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
     L.ske_fbp_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_fbp_profile_reset.argtypes = [vp]
+    L.ske_learned_create.argtypes = [vp, P(vp)]
+    L.ske_learned_destroy.argtypes = [vp]
+    L.ske_learned_param_count.argtypes = [vp, i32p]
+    L.ske_learned_init.argtypes = [vp, ctypes.c_ulonglong]
+    L.ske_learned_set_params.argtypes = [vp, vp]
+    L.ske_learned_get_params.argtypes = [vp, vp]
+    L.ske_learned_reconstruct.argtypes = [vp, ctypes.c_int]
+    L.ske_learned_gradient.argtypes = [vp, ctypes.c_int, f64p, vp]
+    L.ske_learned_train_step.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, f64p]
+    L.ske_learned_download.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_random_ellipses.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_learned_render.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_learned_make_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_float,
+                                         ctypes.c_float]
     _lib = L
     return L
 

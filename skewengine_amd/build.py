@@ -21,8 +21,9 @@ LIB = os.path.join(HERE, "libskewengine.so")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SKE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_core.hip", "kernels_sdem.hip", "kernels_radn.hip", "kernels_blur.hip", "kernels_fbp.hip"]
-CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "driver.cpp"]
+HIP_SOURCES = ["kernels_core.hip", "kernels_sdem.hip", "kernels_radn.hip", "kernels_blur.hip", "kernels_fbp.hip",
+               "kernels_learned.hip"]
+CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "learned.cpp", "driver.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-I" + INCLUDE, "-I" + CSRC]

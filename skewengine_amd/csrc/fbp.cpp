@@ -9,40 +9,13 @@
 #include <memory>
 #include <vector>
 
-#include "fbp.h"
-#include "host_util.h"
-#include "skewengine.h"
+#include "fbp_object.h"
 
 using namespace ske;
 
-struct ske_fbp {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    FbpGeom g{};
-    int max_batch = 0;
-    DevBuf<float2> cs;
-    DevBuf<float> taps;
-    std::vector<float> taps_host;
-    DevBuf<float> buf[4];  // image, sinogram, filtered, reconstruction
-    LaunchTimer timer;
-    ~ske_fbp() {
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
-    size_t elems(int which, int batch) const {
-        const size_t per = (which == SKE_FBP_IMAGE || which == SKE_FBP_RECON) ? (size_t)g.n * g.n
-                                                                              : (size_t)g.nang * g.ndet;
-        return per * (size_t)batch;
-    }
-};
-
-namespace {
-
 // delta * Ram-Lak: h(0) = 1/(4 delta^2), h(m odd) = -1/(m^2 pi^2 delta^2),
 // h(m even != 0) = 0, stored at index m + ndet - 1 for |m| < ndet.
-void ramlak_taps(int ndet, double delta, std::vector<float>& taps) {
+void ske::ramlak_taps(int ndet, double delta, std::vector<float>& taps) {
     taps.assign(2 * (size_t)ndet - 1, 0.0f);
     for (int m = -(ndet - 1); m <= ndet - 1; m++) {
         double h = 0.0;
@@ -53,6 +26,8 @@ void ramlak_taps(int ndet, double delta, std::vector<float>& taps) {
         taps[m + ndet - 1] = (float)(delta * h);
     }
 }
+
+namespace {
 
 int check_batch(ske_fbp* f, int batch) {
     if (!f) return fail(SKE_EINVAL, "fbp is NULL");
