@@ -129,6 +129,8 @@ int ske_synchronize(ske_engine *e);
 #define SKE_OPT_SDEM_WAVES 5    /* viewshed: waves per workgroup (2, 4, 8) */
 #define SKE_OPT_SDEM_PERSISTENT 6 /* viewshed: 0 one workgroup per row, k>0 persistent grid of k per CU */
 #define SKE_OPT_SDEM_OCCUPANCY 7  /* viewshed persistent grid: 0 default, 8 compiled for 8 waves/SIMD */
+#define SKE_OPT_RADON_LOCAL 8     /* 1: SKE_KERNEL_RADON computes the local transform (pointRadon per
+                                   * cell), the reference's LRT=true (src/kernelRADN.h:212-220) */
 int ske_set_option(ske_engine *e, int option, long long value);
 /* Accumulated HIP-event time of the launches named `name` since the last
  * reset (profiling option on): total milliseconds and launch count. */

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libske_oracle.so")
 
 SDEM, IDENTITYDEM, RADON, IDENTITYIMG, CEPSTRUM = 0, 1, 2, 3, 4
+RADON_LOCAL = 5  # the Radon operator with the reference's LRT flag set
 
 
 class Params(ctypes.Structure):
@@ -57,6 +58,8 @@ def lib():
         L.orc_blur_parameter2.argtypes = [f64p, ctypes.c_int, i32p]
         L.orc_line_cepstrum.restype = ctypes.c_float
         L.orc_line_cepstrum.argtypes = [ctypes.c_int, f32p, ctypes.c_int, ctypes.c_int]
+        L.orc_point_radon.restype = ctypes.c_double
+        L.orc_point_radon.argtypes = [P(Params), u16p, u16p, f32p, ctypes.c_int, ctypes.c_int, f64p]
         L.orc_sdem_observer.restype = ctypes.c_float
         L.orc_sdem_observer.argtypes = [f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, u8p, u8p]
         L.orc_default_threads.restype = ctypes.c_int
@@ -164,6 +167,16 @@ def sdem_observer(row, first, last, j, povh):
     cv = lib().orc_sdem_observer(_p(row, ctypes.c_float), first, last, j, np.float32(povh),
                                  _p(vr, ctypes.c_uint8), _p(vl, ctypes.c_uint8))
     return cv, vr[:max(last - j - 1, 0)], vl[:max(j - first, 0)]
+
+
+def point_radon(sec, i, j):
+    """pointRadon (src/kernelRADN.h:114-182) of skewed cell (i, j) of a
+    sector() result: (value, the 64 squared magnitudes the peak rule scans)."""
+    spec = np.zeros(64, np.float64)
+    v = lib().orc_point_radon(ctypes.byref(sec["p"]), _p(sec["first"], ctypes.c_uint16),
+                              _p(sec["last"], ctypes.c_uint16), _p(sec["S"], ctypes.c_float), i, j,
+                              _p(spec, ctypes.c_double))
+    return v, spec
 
 
 def default_threads():

@@ -378,6 +378,59 @@ float orc_line_cepstrum(int j, const float *row, int first, int last)
     return (float)(val * 10000);
 }
 
+/* kernelRADN.h:114-182 (pointRadon, local Radon transform, LRT=true) for
+ * skewed cell (i, j): 64 rows k = i-32..i+31 of 65-column window sums
+ * (columns j-32..j+32 inside [first[k], last[k]), zero for rows outside
+ * [0, H)) times a Gaussian of sigma = (64/5)/scale, FFT/8, a sine ramp on
+ * the real parts of bins 0..25 (zero above 25.6), FFT/8 again, and the
+ * squared magnitude at the first rise after bin 5 (getParameter :96-108). */
+double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                       int i, int j, double *spec_out)
+{
+    const int w = p->skewWidth, h = p->skewHeight, w2 = ORC_WIN / 2;
+    const double dws = ORC_WIN;
+    double sigma = ORC_WIN / 5.0;
+    sigma = sigma / p->scale;
+    double re[ORC_WIN], im[ORC_WIN];
+    for (int k = i - w2; k < i + w2; k++) {
+        double accum = 0;
+        if (k >= 0 && k < h) {
+            int fi = first[k], la = last[k];
+            for (int l = j - w2; l <= j + w2; l++)
+                accum += (l < fi || l >= la) ? 0 : S[(size_t)k * w + l];
+        }
+        int kk = k - (i - w2);
+        double gauss = exp(-(kk - w2) * (kk - w2) / (2 * sigma * sigma));
+        re[kk] = accum * gauss;
+        im[kk] = 0.0;
+    }
+    orc_fft(re, im, ORC_WIN, 1);
+    double cutoff = 0.4;
+    double re2[ORC_WIN], im2[ORC_WIN];
+    for (int k = 0; k < ORC_WIN; k++) {
+        re2[k] = k > cutoff * dws ? 0 : sin((k / (cutoff * dws - 1)) * (M_PI / 2)) * re[k];
+        im2[k] = 0.0;
+    }
+    orc_fft(re2, im2, ORC_WIN, 1);
+    if (spec_out)
+        for (int k = 0; k < ORC_WIN; k++) spec_out[k] = re2[k] * re2[k] + im2[k] * im2[k];
+    int n = w2, ii;
+    for (ii = 5; ii < n; ii++)
+        if (re2[ii] * re2[ii] + im2[ii] * im2[ii] > re2[ii - 1] * re2[ii - 1] + im2[ii - 1] * im2[ii - 1]) break;
+    if (ii == n) return 0.0;
+    return re2[ii - 1] * re2[ii - 1] + im2[ii - 1] * im2[ii - 1];
+}
+
+/* kernelRADN.h:214-220 (radon with LRT=true). */
+void orc_op_radon_local(const orc_params *p, const uint16_t *first, const uint16_t *last,
+                        const float *S, float *SO)
+{
+    for (int i = 0; i < p->skewHeight; i++) {
+        size_t k = (size_t)p->skewWidth * i;
+        for (int j = first[i]; j < last[i]; j++) SO[k + j] = (float)orc_point_radon(p, first, last, S, i, j, NULL);
+    }
+}
+
 /* kernelBLUR.h:267-302 (cepstrum). */
 void orc_op_cepstrum(const orc_params *p, const uint16_t *first, const uint16_t *last,
                      const float *S, float *SO)
@@ -449,6 +502,7 @@ static void eng_sector(orc_engine *e, int alg, int angle, float povh, int do_des
     switch (alg) {
     case ORC_SDEM: orc_op_sdem(p, e->first, e->last, e->S, e->SO, povh); break;
     case ORC_RADON: orc_op_radon(p, e->first, e->last, e->S, e->SO); break;
+    case ORC_RADON_LOCAL: orc_op_radon_local(p, e->first, e->last, e->S, e->SO); break;
     case ORC_CEPSTRUM: orc_op_cepstrum(p, e->first, e->last, e->S, e->SO); break;
     default: orc_op_identity(p, e->first, e->last, e->S, e->SO); break;
     }

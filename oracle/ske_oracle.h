@@ -35,6 +35,9 @@ typedef struct orc_params {
 } orc_params;
 
 enum { ORC_SDEM = 0, ORC_IDENTITYDEM = 1, ORC_RADON = 2, ORC_IDENTITYIMG = 3, ORC_CEPSTRUM = 4 };
+/* Oracle-only id: the Radon operator with the reference's LRT flag set
+ * (local transform, kernelRADN.h:212-220). */
+enum { ORC_RADON_LOCAL = 5 };
 
 float orc_fangle(int dimx, int dimy);
 void orc_compute_params(int dimx, int dimy, int angle, orc_params *p, int *target, float *weight);
@@ -55,6 +58,12 @@ void orc_op_radon(const orc_params *p, const uint16_t *first, const uint16_t *la
                   const float *S, float *SO);
 void orc_op_cepstrum(const orc_params *p, const uint16_t *first, const uint16_t *last,
                      const float *S, float *SO);
+void orc_op_radon_local(const orc_params *p, const uint16_t *first, const uint16_t *last,
+                        const float *S, float *SO);
+/* One cell of the local transform; spec_out (64 doubles, may be NULL)
+ * receives the squared magnitudes the peak rule scans. */
+double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                       int i, int j, double *spec_out);
 double orc_blur_parameter2(const double *v, int n, int *imax);
 float orc_line_cepstrum(int j, const float *row, int first, int last);
 

@@ -13,6 +13,7 @@ SKE_OK = 0
 KERNEL_SDEM, KERNEL_IDENTITYDEM, KERNEL_RADON, KERNEL_IDENTITYIMG, KERNEL_CEPSTRUM = 0, 1, 2, 3, 4
 OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE, OPT_GRAPH, OPT_SDEM_WAVES, OPT_SDEM_PERSISTENT = 1, 2, 3, 4, 5, 6
 OPT_SDEM_OCCUPANCY = 7
+OPT_RADON_LOCAL = 8
 LEARNED_TAPS = 19105
 LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180

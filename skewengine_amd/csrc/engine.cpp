@@ -89,6 +89,7 @@ struct Chunk {
 struct RunPlan {
     int s0 = 0, s1 = 0, order = 0;
     long long budget = 0;
+    int cell_bytes = 4;
     std::vector<std::unique_ptr<Chunk>> chunks;
     long long max_so = 0, max_q = 0;
     int maxH = 0;
@@ -133,9 +134,11 @@ struct Core {
     DevBuf<double> tw;
     DevBuf<int> tw_off;
     DevBuf<double> ctab;
+    DevBuf<double> gtab;  // local Radon Gaussian windows
     DevBuf<int> counter;
     // options
     int sdem_variant = 0;
+    bool radon_local = false;  // the reference's LRT flag (src/kernelRADN.h:212)
     SdemTuning tune;
     long long arena_bytes = 48LL << 30;
     bool profile = false;
@@ -148,6 +151,7 @@ struct Core {
     // planned runs
     std::unique_ptr<RunPlan> plan;
     DevBuf<float> arena;
+    DevBuf<double> W;  // local Radon window sums, one double per arena cell
     DevBuf<double> P;
     DevBuf<float> q;
     // captured graph of the last all-sector run
@@ -236,6 +240,9 @@ struct Core {
         std::vector<double> c;
         cepstrum_table(c);
         SKE_TRY(ctab.upload(c.data(), c.size(), stream));
+        std::vector<double> gt;
+        lrt_gauss_table(gt);
+        SKE_TRY(gtab.upload(gt.data(), gt.size(), stream));
         SKE_TRY(counter.ensure(16));
         SKE_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
         return SKE_OK;
@@ -274,17 +281,20 @@ struct Core {
         return sync();
     }
 
-    // Plan (or reuse) the chunks of sectors [s0,s1) of every grid.
-    int ensure_plan(int s0, int s1, int order) {
-        if (plan && plan->s0 == s0 && plan->s1 == s1 && plan->order == order && plan->budget == arena_bytes)
+    // Plan (or reuse) the chunks of sectors [s0,s1) of every grid; `budget`
+    // bytes of device memory per chunk at `cell_bytes` per skewed cell.
+    int ensure_plan(int s0, int s1, int order, long long budget, int cell_bytes) {
+        if (plan && plan->s0 == s0 && plan->s1 == s1 && plan->order == order && plan->budget == budget &&
+            plan->cell_bytes == cell_bytes)
             return SKE_OK;
         drop_graph();
         std::unique_ptr<RunPlan> rp(new RunPlan());
         rp->s0 = s0;
         rp->s1 = s1;
         rp->order = order;
-        rp->budget = arena_bytes;
-        const long long budget_floats = std::max<long long>(arena_bytes / 4, 1);
+        rp->budget = budget;
+        rp->cell_bytes = cell_bytes;
+        const long long budget_floats = std::max<long long>(budget / cell_bytes, 1);
         std::unique_ptr<Chunk> cur(new Chunk());
         SectorPlan sp;
         auto flush = [&]() -> int {
@@ -343,8 +353,13 @@ struct Core {
         if (imgs.empty()) return fail(SKE_ESTATE, "no grids");
         SKE_HIP(hipSetDevice(device));
         const int order = (kernel_id == SKE_KERNEL_SDEM) ? kOrderCostDesc : kOrderNatural;
-        SKE_TRY(ensure_plan(s0, s1, order));
-        if (kernel_id == SKE_KERNEL_RADON) {
+        const bool local = kernel_id == SKE_KERNEL_RADON && radon_local;
+        SKE_TRY(ensure_plan(s0, s1, order, arena_bytes, local ? 12 : 4));
+        if (local) {
+            if (arena.n < (size_t)plan->max_so || W.n < (size_t)plan->max_so) drop_graph();
+            SKE_TRY(arena.ensure((size_t)plan->max_so));
+            SKE_TRY(W.ensure((size_t)plan->max_so));
+        } else if (kernel_id == SKE_KERNEL_RADON) {
             if (next_pow2_log(plan->maxH) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
             if (P.n < (size_t)plan->max_q || q.n < (size_t)plan->max_q) drop_graph();
             SKE_TRY(P.ensure((size_t)plan->max_q));
@@ -359,7 +374,8 @@ struct Core {
     // Enqueue the whole all-sector run (no allocation, no synchronisation:
     // safe to capture).
     int enqueue_run(int kernel_id, float povh) {
-        const bool radon = kernel_id == SKE_KERNEL_RADON;
+        const bool local = kernel_id == SKE_KERNEL_RADON && radon_local;
+        const bool radon = kernel_id == SKE_KERNEL_RADON && !local;
         const int is_max = kernel_id == SKE_KERNEL_CEPSTRUM ? 1 : 0;
         const size_t acc_n = acc.n;
         SKE_TRY(launch("fill", [&] { return launch_fill_f32(acc.p, acc_n, 0.0f, stream); }));
@@ -382,6 +398,13 @@ struct Core {
                 SKE_TRY(launch("op_blur", [&] { return launch_op_blur(v, in4.p, nullptr, arena.p, ctab.p, stream); }));
                 break;
             case SKE_KERNEL_RADON:
+                if (local) {
+                    SKE_TRY(launch("fill", [&] { return launch_fill_f64(W.p, (size_t)ip.so_total, stream); }));
+                    SKE_TRY(launch("op_lrt", [&] {
+                        return launch_lrt(v, in4.p, nullptr, W.p, arena.p, ctab.p, gtab.p, stream);
+                    }));
+                    break;
+                }
                 SKE_TRY(launch("fill", [&] { return launch_fill_f64(P.p, (size_t)ip.q_total, stream); }));
                 SKE_TRY(launch("radn_rowsum", [&] { return launch_radn_rowsum(v, in4.p, nullptr, P.p, stream); }));
                 SKE_TRY(launch("radn_filter", [&] {
@@ -417,8 +440,8 @@ struct Core {
         char key[160];
         unsigned pbits = 0;
         memcpy(&pbits, &povh, sizeof pbits);
-        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%lld", kernel_id, s0, s1, pbits, sdem_variant,
-                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, arena_bytes);
+        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%lld:%d", kernel_id, s0, s1, pbits, sdem_variant,
+                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, arena_bytes, radon_local ? 1 : 0);
         if (!gexec || gkey != key) {
             drop_graph();
             SKE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
@@ -471,6 +494,10 @@ struct Core {
         case SKE_OPT_SDEM_OCCUPANCY:
             if (value != 0 && value != 8) return fail(SKE_EINVAL, "occupancy hint must be 0 or 8");
             tune.occupancy = (int)value;
+            return SKE_OK;
+        case SKE_OPT_RADON_LOCAL:
+            radon_local = value != 0;
+            drop_graph();
             return SKE_OK;
         default:
             return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
@@ -554,6 +581,7 @@ struct ske_engine {
     DevBuf<uint16_t> first1, last1;
     DevBuf<double> P1;
     DevBuf<float> q1;
+    DevBuf<double> W1;  // local Radon window sums of the current sector
 };
 
 namespace {
@@ -635,6 +663,14 @@ int sector_kernel(ske_engine* e) {
         SKE_TRY(c.launch("op_blur", [&] { return launch_op_blur(v, c.in4.p, e->S.p, e->SO.p, c.ctab.p, c.stream); }));
         break;
     case SKE_KERNEL_RADON: {
+        if (c.radon_local) {
+            SKE_TRY(e->W1.ensure(2 * (size_t)e->N + e->dim_l));
+            SKE_TRY(c.launch("fill", [&] { return launch_fill_f64(e->W1.p, (size_t)sp.H * sp.dim_i, c.stream); }));
+            SKE_TRY(c.launch("op_lrt", [&] {
+                return launch_lrt(v, c.in4.p, e->S.p, e->W1.p, e->SO.p, c.ctab.p, c.gtab.p, c.stream);
+            }));
+            break;
+        }
         if (next_pow2_log(sp.H) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
         SKE_HIP(hipMemsetAsync(e->P1.p, 0, (size_t)sp.H * sizeof(double), c.stream));
         SKE_TRY(c.launch("radn_rowsum", [&] { return launch_radn_rowsum(v, c.in4.p, e->S.p, e->P1.p, c.stream); }));
@@ -660,6 +696,9 @@ int sector_deskew(ske_engine* e, int is_max) {
     int range[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     range[2 * t] = 0;
     range[2 * t + 1] = 1;
+    if (!e->present[t]) {  // zeroed on first use, like the reference's O buffers (src/skewEngine.cuh:759-765)
+        SKE_TRY(c.launch("fill", [&] { return launch_fill_f32(c.acc.p + (size_t)t * e->N, (size_t)e->N, 0.0f, c.stream); }));
+    }
     SKE_TRY(c.launch("deskew", [&] {
         return launch_deskew(v, range, e->SO.p, kDeskewDense, is_max ? 1 : 0, c.acc.p, e->dimx, e->dimy, 0, c.stream);
     }));
@@ -795,6 +834,9 @@ int ske_run(ske_engine* e, int kernel_id, int sector_begin, int sector_end, floa
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
     Core& c = e->core;
     SKE_TRY(c.run(kernel_id, sector_begin, sector_end, povh));
+    // the run reused the orientation accumulators: the per-sector path
+    // starts over from zeroed ones
+    for (int t = 0; t < 4; t++) e->present[t] = 0;
     if (output) {
         SKE_HIP(hipMemcpyAsync(output, c.out.p, (size_t)e->N * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         return c.sync();

@@ -1,5 +1,6 @@
-// kernels_radn.hip -- global Radon operator (src/kernelRADN.h:221-258,
-// LRT=false) on gfx950.
+// kernels_radn.hip -- Radon operator (src/kernelRADN.h:212-258) on gfx950:
+// the global transform (LRT=false) and the local one (LRT=true, pointRadon
+// :114-182).
 //
 //   rowsum : one wave per skewed row, P_r = sum_j S(r,j) in double (the
 //            reference sums in double too, kernelRADN.h:227-232).
@@ -146,6 +147,108 @@ __global__ __launch_bounds__(256) void k_radn_smear(const SecDesc* __restrict__ 
 hipError_t launch_radn_smear(const DevPlan& p, const float* q, float* SO, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
     hipLaunchKernelGGL(k_radn_smear, dim3((p.nwork + 3) / 4), dim3(256), 0, s, p.secs, p.work, p.nwork, p.fl, q, SO);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Local Radon transform (LRT=true).  For skewed cell (i, j) the reference
+// (pointRadon, kernelRADN.h:114-182) forms 64 samples
+//   x_kk = g_kk * sum_{l=j-32}^{j+32} S(k, l) [first_k <= l < last_k],
+//   k = i - 32 + kk (zero for k outside [0, H)),
+// g_kk = exp(-(kk-32)^2 / (2 sigma^2)), sigma = (64/5)/scale, then
+//   X = FFT64(x)/8,  y_f = sin(f/24.6 * pi/2) Re X_f for f <= 25, 0 above,
+//   Y = FFT64(y)/8,  out = |Y_{m-1}|^2 at the first m in [5, 32) with
+//   |Y_m|^2 > |Y_{m-1}|^2 (0 if none; getParameter :96-108).
+// Here in two passes:
+//   lrt_rowsum: one workgroup per skewed row; the row is staged in LDS and
+//     every column's 65-wide window sum is formed in double in the
+//     reference's order (ascending l), for all dim_i columns -> W.
+//   op_lrt: one wave per skewed row, lanes over its cells.  Each cell reads
+//     its 64 window sums down column j of W (coalesced across the lanes),
+//     and evaluates the 26 needed real parts of X and then |Y_m|^2 for
+//     m = 4, 5, ... directly as cosine/sine sums (FP64), stopping at the
+//     first rise.  The direct sums differ from the reference's radix-2 FFT
+//     only by FP64 rounding.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(128) void k_lrt_rowsum(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
+                                                    const float* __restrict__ in4, const int* __restrict__ target,
+                                                    const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                                                    const float* __restrict__ Sbuf, double* __restrict__ W) {
+    extern __shared__ float lds[];
+    float* row = lds + kPad;
+    const RowWork w = work[blockIdx.x];
+    const SecDesc sd = secs[w.sec];
+    const uint32_t v = fl[sd.rows + w.row];
+    const int f = fl_first(v);
+    const int L = fl_last(v) - f;
+    if (L <= 0) return;
+    stage_row(row, sd, w.row, f, L, in4, target, weight, Sbuf);
+    __syncthreads();
+    double* out = W + sd.so + (size_t)w.row * sd.dim_i;
+    for (int j = threadIdx.x; j < sd.dim_i; j += blockDim.x) {
+        const int lo = max(j - 32, f) - f, hi = min(j + 32, f + L - 1) - f;
+        double acc = 0.0;
+        for (int l = lo; l <= hi; ++l) acc += (double)row[l];
+        out[j] = acc;
+    }
+}
+
+// tab: cos/sin(2 pi t/64) interleaved at [0,128), the sine ramp
+// sin(f/24.6 * pi/2) for f <= 25 at [160,186); gauss: 64 weights per folded
+// angle (newAngle).
+__global__ __launch_bounds__(64) void k_op_lrt(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
+                                               const uint32_t* __restrict__ fl, const double* __restrict__ W,
+                                               const double* __restrict__ tab, const double* __restrict__ gauss,
+                                               float* __restrict__ arena) {
+    const RowWork w = work[blockIdx.x];
+    const SecDesc sd = secs[w.sec];
+    const int i = w.row;
+    const uint32_t v = fl[sd.rows + i];
+    const int f = fl_first(v), l = fl_last(v);
+    const double* g = gauss + 64 * sd.newAngle;
+    const double* Wb = W + sd.so;
+    const int k0 = max(0, 32 - i), k1 = min(64, sd.H - i + 32);  // kk with 0 <= i-32+kk < H
+    for (int j = f + (int)threadIdx.x; j < l; j += 64) {
+        double re1[26];
+#pragma unroll
+        for (int q = 0; q < 26; ++q) re1[q] = 0.0;
+        for (int kk = k0; kk < k1; ++kk) {
+            const double x = Wb[(size_t)(i - 32 + kk) * sd.dim_i + j] * g[kk];
+#pragma unroll
+            for (int q = 0; q < 26; ++q) re1[q] = __builtin_fma(x, tab[2 * ((q * kk) & 63)], re1[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 26; ++q) re1[q] *= 0.125 * tab[160 + q];
+        double prev = 0.0, val = 0.0;
+        for (int m = 4; m < 32; ++m) {
+            double yr = 0.0, yi = 0.0;
+#pragma unroll
+            for (int q = 0; q < 26; ++q) {
+                const int t = (q * m) & 63;
+                yr = __builtin_fma(re1[q], tab[2 * t], yr);
+                yi = __builtin_fma(re1[q], tab[2 * t + 1], yi);
+            }
+            yr *= 0.125;
+            yi *= 0.125;
+            const double cur = yr * yr + yi * yi;
+            if (m > 4 && cur > prev) {
+                val = prev;
+                break;
+            }
+            prev = cur;
+        }
+        arena[sd.so + (size_t)i * sd.dim_i + j] = (float)val;
+    }
+}
+
+hipError_t launch_lrt(const DevPlan& p, const float* in4, const float* Sbuf, double* W, float* arena,
+                      const double* tab, const double* gauss, hipStream_t s) {
+    if (p.nwork == 0) return hipSuccess;
+    const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPad);
+    hipLaunchKernelGGL(k_lrt_rowsum, dim3(p.nwork), dim3(128), lds, s, p.secs, p.work, in4, p.target, p.weight, p.fl,
+                       Sbuf, W);
+    hipLaunchKernelGGL(k_op_lrt, dim3(p.nwork), dim3(64), 0, s, p.secs, p.work, p.fl, W, tab, gauss, arena);
     return hipGetLastError();
 }
 

@@ -61,6 +61,14 @@ hipError_t launch_radn_rowsum(const DevPlan& p, const float* in4, const float* S
 hipError_t launch_radn_filter(const DevPlan& p, const double* P, float* q, const double* tw,
                               const int* tw_off, int maxH2, hipStream_t s);
 
+// Local Radon transform (LRT=true): window sums of every skewed row into W
+// (doubles at the arena offsets of the rows, zero-filled beforehand by the
+// caller), then the per-cell transform into the arena.  tab as for the
+// blur operator plus the sine ramp at [160,186); gauss: 64 weights per
+// folded angle 0..90.
+hipError_t launch_lrt(const DevPlan& p, const float* in4, const float* Sbuf, double* W, float* arena,
+                      const double* tab, const double* gauss, hipStream_t s);
+
 // q -> skewOutput rows on [first, last) (materialised per-sector path).
 hipError_t launch_radn_smear(const DevPlan& p, const float* q, float* SO, hipStream_t s);
 

@@ -192,12 +192,27 @@ void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off) {
 }
 
 void cepstrum_table(std::vector<double>& tab) {
-    tab.assign(160, 0.0);
+    tab.assign(192, 0.0);
     for (int t = 0; t < 64; t++) {
         tab[2 * t] = cos(2.0 * M_PI * t / 64.0);
         tab[2 * t + 1] = sin(2.0 * M_PI * t / 64.0);
     }
     for (int t = 0; t < 32; t++) tab[128 + t] = cos(2.0 * M_PI * t / 32.0);
+    // src/kernelRADN.h:152-162: cutoff 0.4 of the 64-sample window
+    const double cutoff = 0.4, dws = 64;
+    for (int k = 0; k < 26; k++) tab[160 + k] = sin((k / (cutoff * dws - 1)) * (M_PI / 2));
+}
+
+void lrt_gauss_table(std::vector<double>& tab) {
+    // src/kernelRADN.h:123-138: sigma = winSize/5 / scale, scale = 1/cos(na)
+    const double torads = M_PI / 180.0;
+    tab.assign(91 * 64, 0.0);
+    for (int na = 0; na <= 90; na++) {
+        const double scale = 1 / cos(torads * na);
+        double sigma = 64 / 5.0;
+        sigma = sigma / scale;
+        for (int kk = 0; kk < 64; kk++) tab[64 * na + kk] = exp(-(kk - 32) * (kk - 32) / (2 * sigma * sigma));
+    }
 }
 
 }  // namespace ske
