@@ -202,6 +202,21 @@ int ske_fbp_set_option(ske_fbp *f, int option, long long value); /* SKE_OPT_PROF
 int ske_fbp_profile_get(ske_fbp *f, const char *name, double *total_ms, int *count);
 int ske_fbp_profile_reset(ske_fbp *f);
 
+/* ---- PNG input and result images (src/helper.h:167-186 read_png via
+ * lodepng; src/main.h:124-141 and src/kernelRADN.h:383-401 showResults).
+ * ske_png_read: width/height of a non-interlaced PNG and, when pixels is
+ * not NULL, its grey level R+G+B (0..765) per pixel of the 8-bit RGBA
+ * decode, row-major, as the reference's image operators take their input. */
+int ske_png_read(const char *path, int *width, int *height, float *pixels);
+/* 8-bit grey (channels 1) or RGB (channels 3) PNG, rows top to bottom. */
+int ske_png_write(const char *path, int width, int height, int channels, const unsigned char *data);
+/* salida1.png (input grey level / 3) and salida2.png (the output mapped to
+ * grey: Radon auto-scaled (out - min) * 255/(max - min), identity image
+ * out/540) in directory dir; kernel_id SKE_KERNEL_RADON or
+ * SKE_KERNEL_IDENTITYIMG. */
+int ske_result_pngs(const char *dir, int kernel_id, int width, int height, const float *pixels,
+                    const float *output);
+
 /* ---- Learned sparse-view FBP.  No reference counterpart: it extends the
  * Radon/FBP operator (src/kernelRADN.h:212-259) with a learnable filter and
  * a residual CNN, in the geometry of an ske_fbp whose stream, timer and

@@ -34,7 +34,7 @@ EXPORTS = [
     "ske_learned_create", "ske_learned_destroy", "ske_learned_param_count", "ske_learned_init",
     "ske_learned_set_params", "ske_learned_get_params", "ske_learned_reconstruct", "ske_learned_gradient",
     "ske_learned_train_step", "ske_learned_download", "ske_random_ellipses", "ske_learned_render",
-    "ske_learned_make_batch",
+    "ske_learned_make_batch", "ske_png_read", "ske_png_write", "ske_result_pngs",
 ]
 
 
@@ -120,6 +120,9 @@ def lib():
     L.ske_fbp_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
     L.ske_fbp_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_fbp_profile_reset.argtypes = [vp]
+    L.ske_png_read.argtypes = [ctypes.c_char_p, i32p, i32p, vp]
+    L.ske_png_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    L.ske_result_pngs.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
     L.ske_learned_create.argtypes = [vp, P(vp)]
     L.ske_learned_destroy.argtypes = [vp]
     L.ske_learned_param_count.argtypes = [vp, i32p]

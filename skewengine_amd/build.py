@@ -23,7 +23,7 @@ ARCH = os.environ.get("SKE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels_core.hip", "kernels_sdem.hip", "kernels_radn.hip", "kernels_blur.hip", "kernels_fbp.hip",
                "kernels_learned.hip"]
-CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "learned.cpp", "driver.cpp"]
+CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "learned.cpp", "png_io.cpp", "driver.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-I" + INCLUDE, "-I" + CSRC]
@@ -52,7 +52,7 @@ def build(verbose=False):
         _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-lamdhip64"])
+    _run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-lamdhip64", "-lz"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

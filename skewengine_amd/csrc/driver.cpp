@@ -4,7 +4,11 @@
 //     (readBilData, src/kernelSDEM.h:31-55), square side from the file size
 //     (check_dem, src/kernelSDEM.h:134-136);
 //   * raw float32 grids (.flt), the format the viewshed result is written in
-//     (showResultsSDEM, src/kernelSDEM.h:433-439).
+//     (showResultsSDEM, src/kernelSDEM.h:433-439);
+//   * PNG images, grey level R+G+B per pixel (configureRADN,
+//     src/kernelRADN.h:80-91, read_png src/helper.h:167-186), and the result
+//     images salida1.png / salida2.png of the Radon and identity-image
+//     operators (src/main.h:124-141) with --png-dir.
 // Unlike the reference's main (which hard-codes operator and file,
 // src/main.cu:176-183), the flags are honoured; with no flags the total
 // viewshed runs on a built-in synthetic 500x500 DEM.
@@ -106,7 +110,8 @@ int kernel_from_name(const char* s) {
 
 extern "C" int run(int argc, char* argv[]) {
     int kernel = SKE_KERNEL_SDEM;
-    std::string input, output;
+    int local = 0;
+    std::string input, output, png_dir;
     int dimx = 0, dimy = 0, s0 = 0, s1 = SKE_MAX_SECTOR, device = 0;
     float povh = (float)(1.5 / 10.0);  // POVh = obsheight/step (src/kernelSDEM.h:159)
     for (int i = 1; i < argc; i++) {
@@ -133,6 +138,11 @@ extern "C" int run(int argc, char* argv[]) {
         } else if (!strcmp(a, "--device") && v) {
             device = atoi(v);
             i++;
+        } else if (!strcmp(a, "--png-dir") && v) {
+            png_dir = v;
+            i++;
+        } else if (!strcmp(a, "--local")) {  // the reference's LRT flag (src/kernelRADN.h:212)
+            local = 1;
         } else {
             fprintf(stderr, "run: unknown or incomplete argument '%s'\n", a);
             return SKE_EINVAL;
@@ -146,6 +156,16 @@ extern "C" int run(int argc, char* argv[]) {
     if (input.empty()) {
         dimx = dimy = 500;
         synthetic_dem(500, grid);
+    } else if (ends_with(input, ".png") || ends_with(input, ".PNG")) {
+        int rc = ske_png_read(input.c_str(), &dimx, &dimy, nullptr);
+        if (rc == SKE_OK) {
+            grid.assign((size_t)dimx * dimy, 0.0f);
+            rc = ske_png_read(input.c_str(), &dimx, &dimy, grid.data());
+        }
+        if (rc != SKE_OK) {
+            fprintf(stderr, "run: %s\n", ske_last_error());
+            return rc;
+        }
     } else if (ends_with(input, ".bil")) {
         const long sz = file_size(input.c_str());
         if (sz <= 0) {
@@ -175,7 +195,8 @@ extern "C" int run(int argc, char* argv[]) {
     ske_engine* e = nullptr;
     int rc = ske_create(dimx, dimy, grid.data(), device, &e);
     if (rc == SKE_OK) {
-        rc = ske_run(e, kernel, s0, s1, povh, out.data());
+        if (local) rc = ske_set_option(e, SKE_OPT_RADON_LOCAL, 1);
+        if (rc == SKE_OK) rc = ske_run(e, kernel, s0, s1, povh, out.data());
         ske_destroy(e);
     }
     auto t2 = std::chrono::steady_clock::now();
@@ -198,6 +219,13 @@ extern "C" int run(int argc, char* argv[]) {
             return SKE_EINVAL;
         }
         fclose(f);
+    }
+    if (!png_dir.empty()) {
+        rc = ske_result_pngs(png_dir.c_str(), kernel, dimx, dimy, grid.data(), out.data());
+        if (rc != SKE_OK) {
+            fprintf(stderr, "run: %s\n", ske_last_error());
+            return rc;
+        }
     }
     return SKE_OK;
 }
