@@ -195,6 +195,39 @@ def extra_c3_cepstrum(ske, synth, count):
             "op_blur_TFLOPs_fp64_at_2848_per_cell": 2848 * total / (k_ms / 1e3) / 1e12}
 
 
+def extra_c5_learned(batch=64):
+    """Config 5: learned sparse-view FBP on 512^2 phantoms, 60 angles, 725
+    bins, Poisson noise at I0 = 1e5, batch 64: time of plain Ram-Lak FBP vs
+    the learned reconstruction (filter + back-projection + CNN) on the same
+    sinograms, and held-out PSNR of both with the committed parameters."""
+    from skewengine_amd import learned as lm
+    from skewengine_amd.fbp import FBP
+    path = lm.default_weights(512, 60, 725)
+    res = {"workload": "%d x 512^2 ellipse phantoms, 60 angles x 725 bins, Poisson I0=1e5, batch inference" % batch}
+    with FBP(512, 60, 725, max_batch=batch) as f, lm.LearnedFBP(f) as m:
+        if path:
+            m.load(path)
+        res["weights"] = os.path.basename(path) if path else None
+        m.make_batch(batch, seed=lm.HELD_OUT_SEED)
+        t_fbp = _timed(lambda: f.reconstruct(batch), f.synchronize, 10)
+        t_l = _timed(lambda: m.reconstruct(batch), f.synchronize, 10)
+        f.set_profile(True)
+        f.profile_reset()
+        m.reconstruct(batch)
+        f.synchronize()
+        cnn_ms = f.profile("cnn_forward")[0]
+        f.set_profile(False)
+        flop = 2 * 9 * (1 * 32 + 32 * 32 + 32 * 32 + 32 * 1) * 512 * 512 * batch
+        res.update({"fbp_ms_per_slice": t_fbp * 1e3 / batch, "learned_ms_per_slice": t_l * 1e3 / batch,
+                    "time_ratio": t_l / t_fbp, "cnn_ms_per_slice": cnn_ms / batch,
+                    "cnn_TFLOPs": flop / (cnn_ms / 1e3) / 1e12})
+        if path:
+            p_fbp, p_l = m.evaluate(64, batch)
+            res.update({"psnr_fbp_db": float(p_fbp.mean()), "psnr_learned_db": float(p_l.mean()),
+                        "psnr_gain_db": float((p_l - p_fbp).mean()), "held_out": 64})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,9 +235,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations 1-3")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations 1, 2, 3 and 5")
     ap.add_argument("--cepstrum-images", type=int, default=64)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "copies": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU only (the sector farm shards across GPUs only by design); "
@@ -225,7 +270,8 @@ def main():
     if not args.no_extra:
         line["extra"] = {"c1_viewshed_500": extra_c1_viewshed(ske, synth),
                          "c2_radon_fbp_512": extra_c2_radon(ske, synth),
-                         "c3_cepstrum_batch": extra_c3_cepstrum(ske, synth, args.cepstrum_images)}
+                         "c3_cepstrum_batch": extra_c3_cepstrum(ske, synth, args.cepstrum_images),
+                         "c5_learned_sparse_fbp": extra_c5_learned()}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_viewshed(steps_total, pts_sectors, args.cpu_seconds)
         line["speedup_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]
