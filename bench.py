@@ -74,10 +74,15 @@ def cpu_baseline_viewshed(total_steps, total_pts_sectors, seconds_budget):
     }
 
 
+def _progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def bench_viewshed_batch(args, ske, synth):
     from skewengine_amd import _lib
     sides, hursts = synth.viewshed_batch_sizes(32, seed=4)
     dems = [synth.dem(n, h, 1000 + i, 3000.0) for i, (n, h) in enumerate(zip(sides, hursts))]
+    _progress("synthesised %d DEMs" % len(dems))
     batch = ske.SkewBatch(dems)  # inputs now resident in HBM
     del dems
     steps_total = 0.0
@@ -87,17 +92,20 @@ def bench_viewshed_batch(args, ske, synth):
         steps_total += st
         pts_sectors += n * n * 180.0
     waves, pers, occ, _ = batch.autotune(ske.KERNEL_SDEM, ske.POVH_DEFAULT)
+    _progress("autotuned: %d waves/workgroup, %d persistent workgroups/CU, occupancy hint %d" % (waves, pers, occ))
     batch.set_option(_lib.OPT_PROFILE, 1)  # HIP events around every launch on the batch stream
 
     for _ in range(args.warmup):
         batch.run(ske.KERNEL_SDEM, 0, 180, ske.POVH_DEFAULT)
-    batch.synchronize()
+        batch.synchronize()
+        _progress("warmup step done")
     batch.profile_reset()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.run(ske.KERNEL_SDEM, 0, 180, ske.POVH_DEFAULT)
     batch.synchronize()
     t1 = time.perf_counter()
+    _progress("timed %d step(s): %.2f s" % (args.steps, t1 - t0))
     sec_per_step = (t1 - t0) / args.steps
     k_ms, k_cnt = batch.profile("op_sdem")
     achieved = OPS_PER_STEP * steps_total * args.steps / (k_ms / 1e3) / 1e12
@@ -237,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary configurations 1, 2, 3 and 5")
     ap.add_argument("--cepstrum-images", type=int, default=64)
+    ap.add_argument("--only-extra", choices=["c1", "c2", "c3", "c5"], default=None,
+                    help="run just one secondary configuration and print its JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -258,6 +268,12 @@ def main():
 
     import skewengine_amd as ske
     from skewengine_amd import synth
+
+    if args.only_extra:
+        fn = {"c1": lambda: extra_c1_viewshed(ske, synth), "c2": lambda: extra_c2_radon(ske, synth),
+              "c3": lambda: extra_c3_cepstrum(ske, synth, args.cepstrum_images), "c5": extra_c5_learned}
+        print(json.dumps({args.only_extra: fn[args.only_extra]()}))
+        return
 
     res, steps_total, pts_sectors = bench_viewshed_batch(args, ske, synth)
     line = {

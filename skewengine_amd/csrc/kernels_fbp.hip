@@ -17,8 +17,6 @@
 #include <math.h>
 #include <stdint.h>
 
-#include <algorithm>
-
 #include "fbp.h"
 
 namespace ske {
@@ -95,7 +93,10 @@ __global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ i
     sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
 }
 
-// One workgroup per sinogram row: the row and the 2*ndet-1 filter taps in LDS.
+// One workgroup per sinogram row: the row and the 2*ndet-1 filter taps in
+// LDS.  A thread owns 4 consecutive outputs and slides a 4-tap register
+// window along h, so each step costs one broadcast read of p, one read of h
+// and 4 FMAs; every output still sums k' = 0, 1, ... in order.
 __global__ __launch_bounds__(256) void k_fbp_filter(const float* __restrict__ sino, float* __restrict__ filt,
                                                     const float* __restrict__ taps, int ndet) {
     extern __shared__ float sm[];
@@ -106,45 +107,86 @@ __global__ __launch_bounds__(256) void k_fbp_filter(const float* __restrict__ si
     for (int i = threadIdx.x; i < ndet; i += blockDim.x) p[i] = src[i];
     for (int i = threadIdx.x; i < 2 * ndet - 1; i += blockDim.x) h[i] = taps[i];
     __syncthreads();
-    for (int k = threadIdx.x; k < ndet; k += blockDim.x) {
-        float acc = 0.0f;
-        const float* hk = h + k + ndet - 1;  // h(k - k') at hk[-k']
-        for (int kp = 0; kp < ndet; ++kp) acc = __builtin_fmaf(p[kp], hk[-kp], acc);
-        filt[row * ndet + k] = acc;
+    for (int k = 4 * (int)threadIdx.x; k < ndet; k += 4 * (int)blockDim.x) {
+        // outputs k..k+3 (those >= ndet are computed and dropped); h index
+        // of output k+m at k' is k + m - k' + ndet - 1, in [0, 2 ndet - 1)
+        // for k + m < ndet; taps past the end read as 0
+        const int top = 2 * ndet - 1;
+        float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+        int hb = k + ndet - 1;  // h index of output k at k' = 0
+        float w0 = h[hb];
+        float w1 = hb + 1 < top ? h[hb + 1] : 0.0f;
+        float w2 = hb + 2 < top ? h[hb + 2] : 0.0f;
+        float w3 = hb + 3 < top ? h[hb + 3] : 0.0f;
+        for (int kp = 0; kp < ndet; ++kp) {
+            const float x = p[kp];
+            acc0 = __builtin_fmaf(x, w0, acc0);
+            acc1 = __builtin_fmaf(x, w1, acc1);
+            acc2 = __builtin_fmaf(x, w2, acc2);
+            acc3 = __builtin_fmaf(x, w3, acc3);
+            w3 = w2;
+            w2 = w1;
+            w1 = w0;
+            --hb;
+            w0 = hb >= 0 ? h[hb] : 0.0f;
+        }
+        float* o = filt + row * ndet;
+        o[k] = acc0;
+        if (k + 1 < ndet) o[k + 1] = acc1;
+        if (k + 2 < ndet) o[k + 2] = acc2;
+        if (k + 3 < ndet) o[k + 3] = acc3;
     }
 }
 
-// One thread per pixel; angles in chunks staged in LDS.
-// chunk: angles staged per pass (chunk * ndet floats <= 64 KB).
+// One thread per pixel, one workgroup per 16 x 16 pixel tile.  A tile
+// projects onto at most 16*sqrt(2) + 2 < 25 consecutive bins of any angle,
+// so per angle only a 32-bin window of the filtered row, starting at
+// floor(projection of the tile centre) - 16, is staged in LDS (zeros outside
+// the detector, which is what the bounds test of the linear interpolation
+// gives); angles go in chunks of up to kBpChunk.
+constexpr int kBpWin = 32;
+constexpr int kBpChunk = 128;
 __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict__ filt, float* __restrict__ img,
-                                                         const float2* __restrict__ cs, FbpGeom g, float scale,
-                                                         int chunk) {
-    extern __shared__ float q[];  // chunk * ndet
-    const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+                                                         const float2* __restrict__ cs, FbpGeom g, float scale) {
+    __shared__ float q[kBpChunk * kBpWin];
+    __shared__ int base[kBpChunk];
+    const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+    const int j = j0 + (threadIdx.x & 15);
+    const int i = i0 + (threadIdx.x >> 4);
     const int b = blockIdx.z;
     const bool in = i < g.n && j < g.n;
     const float x = ((float)j + 0.5f) * g.delta - 1.0f;
     const float y = 1.0f - ((float)i + 0.5f) * g.delta;
+    const float xc = ((float)j0 + 8.0f) * g.delta - 1.0f;
+    const float yc = 1.0f - ((float)i0 + 8.0f) * g.delta;
     const float inv = 1.0f / g.delta;
     const float* fb = filt + (size_t)b * g.nang * g.ndet;
     float acc = 0.0f;
-    for (int a0 = 0; a0 < g.nang; a0 += chunk) {
-        const int na = min(chunk, g.nang - a0);
+    for (int a0 = 0; a0 < g.nang; a0 += kBpChunk) {
+        const int na = min(kBpChunk, g.nang - a0);
         __syncthreads();
-        for (int t = threadIdx.x; t < na * g.ndet; t += blockDim.x) q[t] = fb[(size_t)a0 * g.ndet + t];
+        for (int t = threadIdx.x; t < na; t += blockDim.x) {
+            const float2 t2 = cs[a0 + t];
+            base[t] = (int)floorf((xc * t2.x + yc * t2.y) * inv + g.det_c) - kBpWin / 2;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < na * kBpWin; t += blockDim.x) {
+            const int aa = t / kBpWin;
+            const int k = base[aa] + (t % kBpWin);
+            q[t] = (k >= 0 && k < g.ndet) ? fb[(size_t)(a0 + aa) * g.ndet + k] : 0.0f;
+        }
         __syncthreads();
         if (in) {
             for (int aa = 0; aa < na; ++aa) {
                 const float2 t2 = cs[a0 + aa];
-                const float s = x * t2.x + y * t2.y;
-                const float kf = s * inv + g.det_c;
+                const float sp = x * t2.x + y * t2.y;
+                const float kf = sp * inv + g.det_c;
                 const float fk = floorf(kf);
                 const int k0 = (int)fk;
                 const float w = kf - fk;
-                const float* qa = q + aa * g.ndet;
-                const float v0 = (k0 >= 0 && k0 < g.ndet) ? qa[k0] : 0.0f;
-                const float v1 = (k0 + 1 >= 0 && k0 + 1 < g.ndet) ? qa[k0 + 1] : 0.0f;
+                const int r = aa * kBpWin + k0 - base[aa];  // in [4, 28) (tile radius < 12 bins)
+                const float v0 = q[r];
+                const float v1 = q[r + 1];
                 acc += v0 + w * (v1 - v0);
             }
         }
@@ -170,9 +212,7 @@ hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, 
 hipError_t launch_fbp_backproject(const float* filt, float* img, const float2* cs, const FbpGeom& g, int batch,
                                   float scale, hipStream_t s) {
     dim3 grid((g.n + 15) / 16, (g.n + 15) / 16, batch);
-    const int chunk = std::max(1, std::min(16, 16384 / g.ndet));
-    const size_t lds = sizeof(float) * (size_t)chunk * g.ndet;
-    hipLaunchKernelGGL(k_fbp_backproject, grid, dim3(256), lds, s, filt, img, cs, g, scale, chunk);
+    hipLaunchKernelGGL(k_fbp_backproject, grid, dim3(256), 0, s, filt, img, cs, g, scale);
     return hipGetLastError();
 }
 

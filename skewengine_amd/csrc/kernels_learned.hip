@@ -45,6 +45,7 @@ constexpr int T = 16;  // tile side
 // ---------------------------------------------------------------------------
 
 // conv 1 -> 32 + bias + relu.  Input r0 fp32 (rounded to bf16 as it is read).
+// Each thread writes its pixel's 32 channels as four 16-byte stores.
 __global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ r0, const float* __restrict__ prm,
                                                      uint16_t* __restrict__ a1, int n) {
     __shared__ float xs[(T + 2) * (T + 2)];
@@ -63,16 +64,22 @@ __global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ r
     float x[9];
 #pragma unroll
     for (int t = 0; t < 9; t++) x[t] = xs[(ty + t / 3) * (T + 2) + tx + t % 3];
-    uint16_t* o = a1 + (((size_t)b * n + i0 + ty) * n + j0 + tx) * C;
+    uint4* o = reinterpret_cast<uint4*>(a1 + (((size_t)b * n + i0 + ty) * n + j0 + tx) * C);
 #pragma unroll
-    for (int co = 0; co < C; co += 2) {
-        float s0 = w[C * 9 + co], s1 = w[C * 9 + co + 1];
+    for (int q = 0; q < 4; q++) {
+        uint32_t pk[4];
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-            s0 = __builtin_fmaf(w[co * 9 + t], x[t], s0);
-            s1 = __builtin_fmaf(w[(co + 1) * 9 + t], x[t], s1);
+        for (int h = 0; h < 4; h++) {
+            const int co = 8 * q + 2 * h;
+            float s0 = w[C * 9 + co], s1 = w[C * 9 + co + 1];
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                s0 = __builtin_fmaf(w[co * 9 + t], x[t], s0);
+                s1 = __builtin_fmaf(w[(co + 1) * 9 + t], x[t], s1);
+            }
+            pk[h] = (uint32_t)f2bf(fmaxf(s0, 0.0f)) | ((uint32_t)f2bf(fmaxf(s1, 0.0f)) << 16);
         }
-        *(uint32_t*)(o + co) = (uint32_t)f2bf(fmaxf(s0, 0.0f)) | ((uint32_t)f2bf(fmaxf(s1, 0.0f)) << 16);
+        o[q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     }
 }
 
@@ -157,32 +164,42 @@ __global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict
 }
 
 // conv 32 -> 1 + bias, residual add; optionally the loss and its gradient.
+// The a3 halo stays bf16 in LDS with a 17-word pixel pitch, so the lanes
+// (consecutive pixels) read distinct banks.
+constexpr int kOP = C / 2 + 1;  // LDS pixel pitch in 32-bit words
 __global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict__ a3, const float* __restrict__ r0,
                                                       const float* __restrict__ prm, float* __restrict__ out,
                                                       const float* __restrict__ target, float* __restrict__ dout,
                                                       float inv_count, double* __restrict__ loss, int n) {
     __shared__ float w[9 * C + 1];
-    __shared__ float xs[(T + 2) * (T + 2) * C];
+    __shared__ uint32_t xs[(T + 2) * (T + 2) * kOP];
     __shared__ float red[256];
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
     for (int t = threadIdx.x; t < 9 * C + 1; t += 256) w[t] = t < 9 * C ? rbf(prm[kW4 + t]) : prm[kB4];
     const uint16_t* xb = a3 + (size_t)b * n * n * C;
-    for (int t = threadIdx.x; t < (T + 2) * (T + 2) * C / 2; t += 256) {
-        const int pix = t / (C / 2), c2 = (t % (C / 2)) * 2;
+    for (int t = threadIdx.x; t < (T + 2) * (T + 2) * 4; t += 256) {
+        const int pix = t >> 2, part = t & 3;
         const int ii = i0 + pix / (T + 2) - 1, jj = j0 + pix % (T + 2) - 1;
-        uint32_t v = 0;
-        if (ii >= 0 && ii < n && jj >= 0 && jj < n) v = *(const uint32_t*)(xb + ((size_t)ii * n + jj) * C + c2);
-        xs[pix * C + c2] = bf2f((uint16_t)(v & 0xffff));
-        xs[pix * C + c2 + 1] = bf2f((uint16_t)(v >> 16));
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (ii >= 0 && ii < n && jj >= 0 && jj < n) v = *(const uint4*)(xb + ((size_t)ii * n + jj) * C + 8 * part);
+        uint32_t* d = xs + pix * kOP + 4 * part;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
     }
     __syncthreads();
     const int ty = threadIdx.x / T, tx = threadIdx.x % T;
     float s = w[9 * C];
     for (int tap = 0; tap < 9; tap++) {
-        const float* xp = xs + ((ty + tap / 3) * (T + 2) + tx + tap % 3) * C;
+        const uint32_t* xp = xs + ((ty + tap / 3) * (T + 2) + tx + tap % 3) * kOP;
 #pragma unroll
-        for (int ci = 0; ci < C; ci++) s = __builtin_fmaf(w[tap * C + ci], xp[ci], s);
+        for (int c2 = 0; c2 < C / 2; c2++) {
+            const uint32_t pr = xp[c2];
+            s = __builtin_fmaf(w[tap * C + 2 * c2], __uint_as_float(pr << 16), s);
+            s = __builtin_fmaf(w[tap * C + 2 * c2 + 1], __uint_as_float(pr & 0xffff0000u), s);
+        }
     }
     const size_t p = ((size_t)b * n + i0 + ty) * n + j0 + tx;
     const float o = r0[p] + s;
@@ -240,7 +257,7 @@ __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ 
                                                       float* __restrict__ dzp, int n) {
     extern __shared__ float sm[];
     float* w = sm;               // [t][co][ci]
-    float* ds = sm + 9 * C * C;  // (TY+2)*(T+2)*C
+    float* ds = sm + 9 * C * C;  // (TY+2)*(T+2) pixels at a pitch of C+1 floats (distinct banks per lane)
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * T;
     for (int t = threadIdx.x; t < 9 * C * C; t += 128) {
@@ -251,7 +268,7 @@ __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ 
     for (int t = threadIdx.x; t < (TY + 2) * (T + 2) * C; t += 128) {
         const int pix = t / C, c = t % C;
         const int ii = i0 + pix / (T + 2) - 1, jj = j0 + pix % (T + 2) - 1;
-        ds[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? db[((size_t)ii * n + jj) * C + c] : 0.0f;
+        ds[pix * (C + 1) + c] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? db[((size_t)ii * n + jj) * C + c] : 0.0f;
     }
     __syncthreads();
     const int ty = threadIdx.x / T, tx = threadIdx.x % T;
@@ -259,7 +276,7 @@ __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ 
 #pragma unroll
     for (int ci = 0; ci < C; ci++) acc[ci] = 0.0f;
     for (int tap = 0; tap < 9; tap++) {
-        const float* gp = ds + ((ty + 2 - tap / 3) * (T + 2) + tx + 2 - tap % 3) * C;
+        const float* gp = ds + ((ty + 2 - tap / 3) * (T + 2) + tx + 2 - tap % 3) * (C + 1);
         for (int co = 0; co < C; co++) {
             const float gv = gp[co];
             const float* wr = w + (tap * C + co) * C;
@@ -276,7 +293,7 @@ __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ 
 __global__ __launch_bounds__(128) void k_conv_in_bwd(const float* __restrict__ dz1, const float* __restrict__ dout,
                                                      const float* __restrict__ prm, float* __restrict__ dr0, int n) {
     __shared__ float w[C * 9];
-    __shared__ float ds[(TY + 2) * (T + 2) * C];
+    __shared__ float ds[(TY + 2) * (T + 2) * (C + 1)];  // pixel pitch C+1: distinct banks per lane
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * T;
     for (int t = threadIdx.x; t < C * 9; t += 128) w[t] = rbf(prm[kW1 + t]);
@@ -284,13 +301,13 @@ __global__ __launch_bounds__(128) void k_conv_in_bwd(const float* __restrict__ d
     for (int t = threadIdx.x; t < (TY + 2) * (T + 2) * C; t += 128) {
         const int pix = t / C, c = t % C;
         const int ii = i0 + pix / (T + 2) - 1, jj = j0 + pix % (T + 2) - 1;
-        ds[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? db[((size_t)ii * n + jj) * C + c] : 0.0f;
+        ds[pix * (C + 1) + c] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? db[((size_t)ii * n + jj) * C + c] : 0.0f;
     }
     __syncthreads();
     const int ty = threadIdx.x / T, tx = threadIdx.x % T;
     float s = 0.0f;
     for (int tap = 0; tap < 9; tap++) {
-        const float* gp = ds + ((ty + 2 - tap / 3) * (T + 2) + tx + 2 - tap % 3) * C;
+        const float* gp = ds + ((ty + 2 - tap / 3) * (T + 2) + tx + 2 - tap % 3) * (C + 1);
 #pragma unroll
         for (int co = 0; co < C; co++) s = __builtin_fmaf(w[co * 9 + tap], gp[co], s);
     }
@@ -548,7 +565,7 @@ hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const flo
                                int batch, hipStream_t s) {
     // activation gradients
     hipLaunchKernelGGL(k_conv_out_bwd, SKE_GRID(n, batch), dim3(256), 0, s, dout, B.a3, prm, B.dz3, n);
-    const size_t lmid = sizeof(float) * (9 * C * C + (TY + 2) * (T + 2) * C);
+    const size_t lmid = sizeof(float) * (9 * C * C + (TY + 2) * (T + 2) * (C + 1));
     dim3 g8(n / T, n / TY, batch);
     hipLaunchKernelGGL(k_conv_mid_bwd, g8, dim3(128), lmid, s, B.dz3, B.a2, prm, kW3, B.dz2, n);
     hipLaunchKernelGGL(k_conv_mid_bwd, g8, dim3(128), lmid, s, B.dz2, B.a1, prm, kW2, B.dz1, n);
