@@ -32,11 +32,14 @@ HBM_PEAK_GBS = 8000.0
 
 
 def _traffic(name):
+    """HBM bytes per launch of `name` from the committed rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE passes (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(name)
-    except (OSError, ValueError):
+            rec = json.load(f).get(name)
+        return rec["bytes_per_launch"] if rec else None
+    except (OSError, ValueError, KeyError, TypeError):
         return None
 
 
