@@ -233,7 +233,8 @@ int ske_result_pngs(const char *dir, int kernel_id, int width, int height, const
 typedef struct ske_learned ske_learned;
 #define SKE_LEARNED_TAPS 19105 /* offset of the filter taps in the parameters */
 #define SKE_LEARNED_R0 0       /* fp32 batch x n x n: learned-filter FBP */
-#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 32: hidden activations */
+#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 32: hidden activations of the last
+                                * ske_learned_gradient / train_step (reconstruct keeps them in LDS) */
 #define SKE_LEARNED_A2 2
 #define SKE_LEARNED_A3 3
 #define SKE_LEARNED_DR0 4      /* fp32 batch x n x n: d loss / d r0 (after a gradient) */
@@ -246,7 +247,9 @@ int ske_learned_param_count(ske_learned *l, int *count);
 int ske_learned_init(ske_learned *l, unsigned long long seed);
 int ske_learned_set_params(ske_learned *l, const float *params); /* also resets the optimiser */
 int ske_learned_get_params(ske_learned *l, float *params);
-int ske_learned_reconstruct(ske_learned *l, int batch); /* SINO -> RECON */
+/* SINO -> RECON.  The CNN runs fused (one kernel, activations in LDS);
+ * the result is bit-identical to the per-layer training forward. */
+int ske_learned_reconstruct(ske_learned *l, int batch);
 /* loss = mean((out - IMAGE)^2) over the batch and its gradient with respect
  * to every parameter (grad: param_count floats, or NULL). */
 int ske_learned_gradient(ske_learned *l, int batch, double *loss, float *grad);

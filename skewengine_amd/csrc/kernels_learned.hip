@@ -29,10 +29,14 @@
 
 namespace ske {
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return (uint16_t)(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    bf16x2 v;
+    v.x = (__bf16)lo;
+    v.y = (__bf16)hi;
+    return __builtin_bit_cast(uint32_t, v);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }  // value as bf16
@@ -44,71 +48,103 @@ constexpr int T = 16;  // tile side
 // forward
 // ---------------------------------------------------------------------------
 
-// conv 1 -> 32 + bias + relu.  Input r0 fp32 (rounded to bf16 as it is read).
-// Each thread writes its pixel's 32 channels as four 16-byte stores.
-__global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ r0, const float* __restrict__ prm,
-                                                     uint16_t* __restrict__ a1, int n) {
-    __shared__ float xs[(T + 2) * (T + 2)];
-    __shared__ float w[C * 9 + C];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    const float* img = r0 + (size_t)b * n * n;
-    for (int t = threadIdx.x; t < (T + 2) * (T + 2); t += 256) {
-        const int ii = i0 + t / (T + 2) - 1, jj = j0 + t % (T + 2) - 1;
-        xs[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? rbf(img[(size_t)ii * n + jj]) : 0.0f;
-    }
-    for (int t = threadIdx.x; t < C * 9 + C; t += 256)
-        w[t] = t < C * 9 ? rbf(prm[kW1 + t]) : prm[kB1 + t - C * 9];
-    __syncthreads();
-    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float x[9];
-#pragma unroll
-    for (int t = 0; t < 9; t++) x[t] = xs[(ty + t / 3) * (T + 2) + tx + t % 3];
-    uint4* o = reinterpret_cast<uint4*>(a1 + (((size_t)b * n + i0 + ty) * n + j0 + tx) * C);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        uint32_t pk[4];
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const int co = 8 * q + 2 * h;
-            float s0 = w[C * 9 + co], s1 = w[C * 9 + co + 1];
-#pragma unroll
-            for (int t = 0; t < 9; t++) {
-                s0 = __builtin_fmaf(w[co * 9 + t], x[t], s0);
-                s1 = __builtin_fmaf(w[(co + 1) * 9 + t], x[t], s1);
-            }
-            pk[h] = (uint32_t)f2bf(fmaxf(s0, 0.0f)) | ((uint32_t)f2bf(fmaxf(s1, 0.0f)) << 16);
-        }
-        o[q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-    }
-}
-
-// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout (MFMA operands).
-__global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb2, uint16_t* __restrict__ wb3) {
+// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout, and W1 as
+// bf16 [co][16] (taps 0..8, then zeros): the matrix-core operands.
+__global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb2, uint16_t* __restrict__ wb3,
+                               uint16_t* __restrict__ w1b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 9 * C * C) {
         wb2[i] = f2bf(prm[kW2 + i]);
         wb3[i] = f2bf(prm[kW3 + i]);
     }
+    if (i < 16 * C) {
+        const int co = i / 16, t = i % 16;
+        w1b[i] = t < 9 ? f2bf(prm[kW1 + co * 9 + t]) : (uint16_t)0;
+    }
 }
 
-// conv 32 -> 32 + bias + relu, bf16 NHWC in/out, as an implicit GEMM on the
-// matrix cores: per output row segment of 32 pixels, D[co][px] =
-// sum over 9 taps x 2 channel halves of W_t[co][16 ci] * X_t[16 ci][px]
-// (v_mfma_f32_32x32x16_bf16, fp32 accumulate).  Operand lane layout:
-// A[i][k] at lane i + 32*(k/8), element k%8; B[k][j] at lane j + 32*(k/8);
-// D[i][j] at lane j + 32*((i/4)%2), register 4*(i/8) + i%4.
-// Workgroup: 4 waves over an 8 x 32 pixel tile, each wave two rows; the
-// (8+2) x (32+2) pixel halo is staged in LDS with an 80-byte pixel pitch
-// (conflict-free 16-byte reads).  Each wave keeps all 18 weight operands of
-// its 32 output channels in registers.
+// The convolutions with 32 output channels run on the matrix cores as
+// implicit GEMMs, D[co][px] = sum_k W[co][k] X[k][px] over 32-pixel groups,
+// with v_mfma_f32_32x32x16_bf16 (fp32 accumulate, the bias as the initial
+// accumulator).  Operand lane layout: A[i][k] at lane i + 32*(k/8), element
+// k%8; B[k][j] at lane j + 32*(k/8); D[i][j] at lane j + 32*((i/4)%2),
+// register 4*(i/8) + i%4.
+//   conv 1 -> 32: k = tap (9 taps padded to 16), one MFMA per group;
+//   conv 32 -> 32: k = (tap, 16 channels), 18 MFMAs per group, taps in
+//   order, each with channels 0-15 then 16-31.
+// Activation tiles sit in LDS as bf16 with an 80-byte pixel pitch (32
+// channels + 8 pad: conflict-free 16-byte reads).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-constexpr int kMR = 8;        // tile rows
+constexpr int kMR = 8;        // tile rows of the unfused kernels
 constexpr int kMW = 32;       // tile columns
 constexpr int kMP = 40;       // LDS pixel pitch in bf16 (32 channels + 8 pad)
 constexpr int kMHalo = (kMR + 2) * (kMW + 2);
 
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// The 8 bf16 B-operand entries of conv 1 -> 32 for one pixel: taps 8*hi ..
+// 8*hi+7 of the bf16 image tile `s0` (row pitch w0) around (y, x).
+__device__ __forceinline__ bf16x8 conv_in_operand(const uint16_t* s0, int w0, int y, int x, int hi) {
+    uint32_t e[8];
+    if (hi == 0) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) e[t] = s0[(y + t / 3) * w0 + x + t % 3];
+    } else {
+        e[0] = s0[(y + 2) * w0 + x + 2];
+#pragma unroll
+        for (int t = 1; t < 8; t++) e[t] = 0;
+    }
+    return as_bf16x8(make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16)));
+}
+
+// relu, bf16, and the group's 16 values per lane into an NHWC tile (8-byte
+// stores of 4 consecutive channels); `keep` false writes zeros.
+__device__ __forceinline__ void store_group(uint16_t* dst, const f32x16& acc, bool keep) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint2 v;
+        v.x = pack_bf16(fmaxf(acc[4 * q], 0.0f), fmaxf(acc[4 * q + 1], 0.0f));
+        v.y = pack_bf16(fmaxf(acc[4 * q + 2], 0.0f), fmaxf(acc[4 * q + 3], 0.0f));
+        if (!keep) v = make_uint2(0, 0);
+        *(uint2*)(dst + 8 * q) = v;
+    }
+}
+
+// conv 1 -> 32 + bias + relu.  Input r0 fp32, rounded to bf16 as it is
+// staged.  Workgroup: 4 waves over an 8 x 32 pixel tile, two rows each.
+__global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ r0, const uint16_t* __restrict__ w1b,
+                                                     const float* __restrict__ prm, uint16_t* __restrict__ a1, int n) {
+    __shared__ uint16_t s0[kMHalo];
+    const int b = blockIdx.z;
+    const int i0 = blockIdx.y * kMR, j0 = blockIdx.x * kMW;
+    const float* img = r0 + (size_t)b * n * n;
+    for (int t = threadIdx.x; t < kMHalo; t += 256) {
+        const int ii = i0 + t / (kMW + 2) - 1, jj = j0 + t % (kMW + 2) - 1;
+        s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const bf16x8 w = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
+    float bias[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) bias[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < kMR / 4; rr++) {
+        const int row = wave * (kMR / 4) + rr;
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = bias[r];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, conv_in_operand(s0, kMW + 2, row, l32, hi), acc, 0, 0, 0);
+        store_group(a1 + (((size_t)b * n + i0 + row) * n + j0 + l32) * C + 4 * hi, acc, true);
+    }
+}
+
+// conv 32 -> 32 + bias + relu, bf16 NHWC in/out.  Workgroup: 4 waves over
+// an 8 x 32 pixel tile, each wave two rows; the (8+2) x (32+2) halo in LDS;
+// each wave keeps the 18 weight operands of its 32 output channels in
+// registers.
 __global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wb,
                                                       const float* __restrict__ prm, int boff,
                                                       uint16_t* __restrict__ y, int n) {
@@ -130,8 +166,7 @@ __global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict
     for (int t = 0; t < 9; t++)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint4 v = *(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi);
-            w[2 * t + h] = __builtin_bit_cast(bf16x8, v);
+            w[2 * t + h] = as_bf16x8(*(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi));
         }
     float bias[16];
 #pragma unroll
@@ -148,35 +183,35 @@ __global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict
             const uint16_t* xp = xs + ((row + t / 3) * (kMW + 2) + l32 + t % 3) * kMP + 8 * hi;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const bf16x8 xv = __builtin_bit_cast(bf16x8, *(const uint4*)(xp + 16 * h));
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], xv, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], as_bf16x8(*(const uint4*)(xp + 16 * h)),
+                                                              acc, 0, 0, 0);
             }
         }
-        uint16_t* o = y + (((size_t)b * n + i0 + row) * n + j0 + l32) * C + 4 * hi;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            uint2 v;
-            v.x = (uint32_t)f2bf(fmaxf(acc[4 * q], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[4 * q + 1], 0.0f)) << 16);
-            v.y = (uint32_t)f2bf(fmaxf(acc[4 * q + 2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[4 * q + 3], 0.0f)) << 16);
-            *(uint2*)(o + 8 * q) = v;
-        }
+        store_group(y + (((size_t)b * n + i0 + row) * n + j0 + l32) * C + 4 * hi, acc, true);
     }
 }
 
 // conv 32 -> 1 + bias, residual add; optionally the loss and its gradient.
 // The a3 halo stays bf16 in LDS with a 17-word pixel pitch, so the lanes
-// (consecutive pixels) read distinct banks.
+// (consecutive pixels) read distinct banks.  Channel pairs are summed with
+// v_dot2c_f32_bf16 (weights rounded to bf16), taps in order, channels
+// 0-1, 2-3, ... within a tap.
 constexpr int kOP = C / 2 + 1;  // LDS pixel pitch in 32-bit words
+__device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
+}
+
 __global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict__ a3, const float* __restrict__ r0,
                                                       const float* __restrict__ prm, float* __restrict__ out,
                                                       const float* __restrict__ target, float* __restrict__ dout,
                                                       float inv_count, double* __restrict__ loss, int n) {
-    __shared__ float w[9 * C + 1];
+    __shared__ uint32_t w[9 * C / 2];
     __shared__ uint32_t xs[(T + 2) * (T + 2) * kOP];
     __shared__ float red[256];
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    for (int t = threadIdx.x; t < 9 * C + 1; t += 256) w[t] = t < 9 * C ? rbf(prm[kW4 + t]) : prm[kB4];
+    for (int t = threadIdx.x; t < 9 * C / 2; t += 256) w[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
+    const float b4 = prm[kB4];
     const uint16_t* xb = a3 + (size_t)b * n * n * C;
     for (int t = threadIdx.x; t < (T + 2) * (T + 2) * 4; t += 256) {
         const int pix = t >> 2, part = t & 3;
@@ -191,15 +226,11 @@ __global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict
     }
     __syncthreads();
     const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float s = w[9 * C];
+    float s = b4;
     for (int tap = 0; tap < 9; tap++) {
         const uint32_t* xp = xs + ((ty + tap / 3) * (T + 2) + tx + tap % 3) * kOP;
 #pragma unroll
-        for (int c2 = 0; c2 < C / 2; c2++) {
-            const uint32_t pr = xp[c2];
-            s = __builtin_fmaf(w[tap * C + 2 * c2], __uint_as_float(pr << 16), s);
-            s = __builtin_fmaf(w[tap * C + 2 * c2 + 1], __uint_as_float(pr & 0xffff0000u), s);
-        }
+        for (int c2 = 0; c2 < C / 2; c2++) s = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s);
     }
     const size_t p = ((size_t)b * n + i0 + ty) * n + j0 + tx;
     const float o = r0[p] + s;
@@ -214,6 +245,151 @@ __global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict
             __syncthreads();
         }
         if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
+    }
+}
+
+// The whole CNN forward for inference, fused: one workgroup per 16 x 16
+// output tile computes a1 on the 22 x 22 region around it (from a 24 x 24
+// tile of r0), a2 on 20 x 20, a3 on 18 x 18, then out -- all in LDS, so HBM
+// sees r0 once and out once.  Region pixels outside the image are zero,
+// the zero padding of the next layer.  The arithmetic per value is the
+// unfused kernels' (same MFMA sequence, same FMA order for conv 32 -> 1),
+// so the output is bit-identical to theirs.  8 waves; each layer's pixels
+// go in 32-pixel groups, group g to wave g mod 8.
+constexpr int kFT = 16;          // output tile side
+constexpr int kF0 = kFT + 8;     // r0 region side
+constexpr int kF1 = kFT + 6;     // a1
+constexpr int kF2 = kFT + 4;     // a2
+constexpr int kF3 = kFT + 2;     // a3
+
+// One conv 32 -> 32 layer from the LDS region `in` (side wi) to `out` (side
+// wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
+// groups g and g + 8 together (two independent accumulators), so the loads
+// of one group overlap the matrix-core work of the other; the bias (32
+// floats in LDS) seeds both accumulators.
+__device__ __forceinline__ void fused_acc_init(f32x16& acc, const float* bias, int hi) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 b = *(const float4*)(bias + 8 * q + 4 * hi);
+        acc[4 * q] = b.x;
+        acc[4 * q + 1] = b.y;
+        acc[4 * q + 2] = b.z;
+        acc[4 * q + 3] = b.w;
+    }
+}
+
+__device__ __forceinline__ void fused_mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
+                                                const uint16_t* __restrict__ wb, const float* bias, int wave, int l32,
+                                                int hi) {
+    const int wo = wi - 2, count = wo * wo;
+    bf16x8 w[18];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) w[2 * t + h] = as_bf16x8(*(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi));
+    for (int g0 = wave; g0 * 32 < count; g0 += 16) {
+        const int g1 = g0 + 8;
+        const int p0 = g0 * 32 + l32, p1 = g1 * 32 + l32;
+        const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
+        const int y0 = q0 / wo, x0 = q0 % wo, y1 = q1 / wo, x1 = q1 % wo;
+        const uint16_t* b0 = in + (y0 * wi + x0) * kMP + 8 * hi;
+        f32x16 acc0;
+        fused_acc_init(acc0, bias, hi);
+        if (g1 * 32 < count) {  // two groups (wave-uniform branch)
+            const uint16_t* b1 = in + (y1 * wi + x1) * kMP + 8 * hi;
+            f32x16 acc1;
+            fused_acc_init(acc1, bias, hi);
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int o = ((t / 3) * wi + t % 3) * kMP;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + o + 16 * h));
+                    const bf16x8 v1 = as_bf16x8(*(const uint4*)(b1 + o + 16 * h));
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], v0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], v1, acc1, 0, 0, 0);
+                }
+            }
+            const int iy = oy + y1, ix = ox + x1;
+            if (p1 < count) store_group(out + p1 * kMP + 4 * hi, acc1, iy >= 0 && iy < n && ix >= 0 && ix < n);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int o = ((t / 3) * wi + t % 3) * kMP;
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h],
+                                                                   as_bf16x8(*(const uint4*)(b0 + o + 16 * h)), acc0,
+                                                                   0, 0, 0);
+            }
+        }
+        const int iy = oy + y0, ix = ox + x0;
+        if (p0 < count) store_group(out + p0 * kMP + 4 * hi, acc0, iy >= 0 && iy < n && ix >= 0 && ix < n);
+    }
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_cnn_fused_fwd(const float* __restrict__ r0, const float* __restrict__ prm,
+                                                       const uint16_t* __restrict__ w1b,
+                                                       const uint16_t* __restrict__ wb2,
+                                                       const uint16_t* __restrict__ wb3, float* __restrict__ out,
+                                                       int n) {
+    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kMP];  // a1, then a3
+    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kMP];  // a2
+    __shared__ uint16_t s0[kF0 * kF0];                     // r0 as bf16
+    __shared__ uint32_t w4[9 * C / 2];                     // W4 as bf16 pairs
+    __shared__ __align__(16) float sbias[2 * C];            // b2, b3
+    const int b = blockIdx.z;
+    const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
+    const float* img = r0 + (size_t)b * n * n;
+    for (int t = threadIdx.x; t < kF0 * kF0; t += 512) {
+        const int ii = i0 - 4 + t / kF0, jj = j0 - 4 + t % kF0;
+        s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
+    }
+    for (int t = threadIdx.x; t < 9 * C / 2; t += 512) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
+    if (threadIdx.x < 2 * C) sbias[threadIdx.x] = prm[(threadIdx.x < C ? kB2 : kB3 - C) + threadIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    __syncthreads();
+    {   // a1 on the 22 x 22 region at (i0-3, j0-3)
+        const bf16x8 w = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
+        float bias[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) bias[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
+        const int count = kF1 * kF1;
+        for (int g = wave; g * 32 < count; g += 8) {
+            const int p = g * 32 + l32;
+            const int pc = min(p, count - 1);
+            const int y = pc / kF1, x = pc % kF1;
+            f32x16 acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = bias[r];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, conv_in_operand(s0, kF0, y, x, hi), acc, 0, 0, 0);
+            const int iy = i0 - 3 + y, ix = j0 - 3 + x;
+            if (p < count) store_group(sa + p * kMP + 4 * hi, acc, iy >= 0 && iy < n && ix >= 0 && ix < n);
+        }
+    }
+    __syncthreads();
+    fused_mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, wb2, sbias, wave, l32, hi);
+    __syncthreads();
+    fused_mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, wb3, sbias + C, wave, l32, hi);
+    __syncthreads();
+    if (threadIdx.x < kFT * kFT) {  // conv 32 -> 1 + bias + residual, the order of k_conv_out_fwd
+        const int y = threadIdx.x / kFT, x = threadIdx.x % kFT;
+        float acc = prm[kB4];
+        for (int tap = 0; tap < 9; tap++) {
+            const uint16_t* xp = sa + ((y + tap / 3) * kF3 + x + tap % 3) * kMP;
+#pragma unroll
+            for (int c8 = 0; c8 < 4; c8++) {
+                const uint4 v = *(const uint4*)(xp + 8 * c8);
+                const uint32_t* wp = w4 + tap * (C / 2) + 4 * c8;
+                acc = dot2_bf16(wp[0], v.x, acc);
+                acc = dot2_bf16(wp[1], v.y, acc);
+                acc = dot2_bf16(wp[2], v.z, acc);
+                acc = dot2_bf16(wp[3], v.w, acc);
+            }
+        }
+        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
+        out[p] = img[(size_t)(i0 + y) * n + j0 + x] + acc;
     }
 }
 
@@ -543,16 +719,23 @@ __global__ void k_noise(float* __restrict__ sino, size_t count, float i0, float 
 
 #define SKE_GRID(n, b) dim3((n) / T, (n) / T, (b))
 
-hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_weights, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb2, wb3);
+hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_weights, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb2, wb3, w1b);
+    return hipGetLastError();
+}
+
+// Inference: the fused forward (n a multiple of 16).
+hipError_t launch_cnn_infer(const LearnedBufs& B, const float* prm, int n, int batch, hipStream_t s) {
+    hipLaunchKernelGGL(k_cnn_fused_fwd, dim3(n / kFT, n / kFT, batch), dim3(512), 0, s, B.r0, prm, B.w1b, B.wb2, B.wb3,
+                       B.out, n);
     return hipGetLastError();
 }
 
 // n must be a multiple of 32 (the matrix-core tiles are 32 pixels wide).
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s) {
-    hipLaunchKernelGGL(k_conv_in_fwd, SKE_GRID(n, batch), dim3(256), 0, s, B.r0, prm, B.a1, n);
     const dim3 gm(n / kMW, n / kMR, batch);
+    hipLaunchKernelGGL(k_conv_in_fwd, gm, dim3(256), 0, s, B.r0, B.w1b, prm, B.a1, n);
     hipLaunchKernelGGL(k_conv_mid_fwd, gm, dim3(256), 0, s, B.a1, B.wb2, prm, kB2, B.a2, n);
     hipLaunchKernelGGL(k_conv_mid_fwd, gm, dim3(256), 0, s, B.a2, B.wb3, prm, kB3, B.a3, n);
     const float inv = 1.0f / ((float)n * n * batch);

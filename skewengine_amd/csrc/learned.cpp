@@ -44,6 +44,7 @@ struct ske_learned {
         }
         B.wb2 = wb.p;
         B.wb3 = wb.p + 9 * kLC * kLC;
+        B.w1b = wb.p + 2 * 9 * kLC * kLC;
         (void)batch;
         return B;
     }
@@ -103,8 +104,9 @@ int forward(ske_learned* l, int batch, bool train) {
                                       s);
     }));
     return f->timer.launch(s, "cnn_forward", [&] {
-        hipError_t e = launch_pack_weights(l->prm.p, l->wb.p, l->wb.p + 9 * kLC * kLC, s);
+        hipError_t e = launch_pack_weights(l->prm.p, l->wb.p, l->wb.p + 9 * kLC * kLC, l->wb.p + 2 * 9 * kLC * kLC, s);
         if (e != hipSuccess) return e;
+        if (!train) return launch_cnn_infer(B, l->prm.p, g.n, batch, s);
         if (train) {
             e = launch_fill_f64(l->loss.p, 1, s);
             if (e != hipSuccess) return e;
@@ -176,7 +178,7 @@ int ske_learned_create(ske_fbp* f, ske_learned** out) {
     SKE_TRY(l->prm.ensure(l->nparam));
     SKE_TRY(l->r0.ensure(pix));
     SKE_TRY(l->act.ensure(3 * pix * kLC));
-    SKE_TRY(l->wb.ensure(2 * 9 * kLC * kLC));
+    SKE_TRY(l->wb.ensure(2 * 9 * kLC * kLC + 16 * kLC));
     SKE_TRY(l->full.ensure(2 * (size_t)f->g.ndet - 1));
     SKE_TRY(ske_learned_init(l.get(), 1));
     *out = l.release();

@@ -35,11 +35,15 @@ struct LearnedBufs {
     float* dr0;          // fp32 gradient w.r.t. r0
     const uint16_t* wb2; // bf16 copies of W2, W3 ([co][t][ci])
     const uint16_t* wb3;
+    const uint16_t* w1b; // bf16 W1 as [co][16] (9 taps, 7 zeros)
 };
 
-hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, hipStream_t s);
+hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s);
+// Training forward: every activation stored (n a multiple of 32).
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s);
+// Inference forward, fused in LDS (no activations stored; n a multiple of 16).
+hipError_t launch_cnn_infer(const LearnedBufs& B, const float* prm, int n, int batch, hipStream_t s);
 hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const float* dout, float* grad, int n,
                                int batch, hipStream_t s);
 hipError_t launch_filter_backward(const float* dr0, float* dq, const float* sino, float* dfull, float* dsym,

@@ -45,7 +45,7 @@ def _random_params(m, nd, n, seed):
     return p
 
 
-@pytest.mark.parametrize("n,nang,ndet,batch", [(64, 30, 91, 2), (96, 20, 137, 1)])
+@pytest.mark.parametrize("n,nang,ndet,batch", [(64, 30, 91, 2), (96, 20, 137, 1), (160, 12, 227, 3)])
 def test_forward_layers_match_fp64(n, nang, ndet, batch):
     with FBP(n, nang, ndet, max_batch=batch) as f, lm.LearnedFBP(f) as m:
         prm = _random_params(m, ndet, n, seed=n)
@@ -53,10 +53,13 @@ def test_forward_layers_match_fp64(n, nang, ndet, batch):
         assert np.array_equal(m.params(), prm)
         m.make_batch(batch, seed=7, i0=1e4, mu=1.0)
         sino = f.download(SINO, batch)
-        m.reconstruct(batch)
+        m.gradient(batch)  # the per-layer forward, which keeps the activations
         r0 = m.download(lm.R0, batch)
         a = [m.download(w, batch) for w in (lm.A1, lm.A2, lm.A3)]
         out = f.download(RECON, batch)
+        m.reconstruct(batch)  # fused forward
+        fused = f.download(RECON, batch)
+    assert np.array_equal(fused, out)
     p64 = prm.astype(np.float64)
     assert _rel(r0, L.learned_fbp(p64, sino, n)) <= 1e-4
     g = L.unpack(p64)
