@@ -164,6 +164,14 @@ def extra_c2_radon(ske, synth):
         t = _timed(lambda: e.run(ske.KERNEL_RADON, copy_out=False), e.synchronize, 20)
     res["skew_radon_fbp"] = {"workload": "512x512 phantom, 180 skew sectors (kernelRADN semantics)",
                              "ms_per_slice": t * 1e3, "slices_per_s": 1.0 / t}
+    from skewengine_amd import _lib
+    cells, _ = ske.work(512, 512)
+    with ske.SkewEngine(img) as e:
+        e.set_option(_lib.OPT_RADON_LOCAL, 1)
+        t = _timed(lambda: e.run(ske.KERNEL_RADON, copy_out=False), e.synchronize, 3)
+    res["local_radon"] = {"workload": "512x512 phantom, 180 skew sectors, local transform (pointRadon, LRT=true)",
+                          "ms_per_slice": t * 1e3, "cells_per_s": cells / t,
+                          "fp64_TFLOPs_if_6240_per_cell": 6240 * cells / t / 1e12}
     batch = 16
     imgs = np.stack([synth.ellipse_phantom(512, seed=100 + i) for i in range(batch)])
     with FBP(512, 180, 725, max_batch=batch) as f:
