@@ -16,8 +16,10 @@ struct FbpGeom {
     float pad;
 };
 
-hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const FbpGeom& g, int batch,
-                              hipStream_t s);
+// ang_bins: angles with |cos| >= |sin| (lanes over bins); ang_rays: the
+// others (lanes over the samples of one ray).
+hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const int* ang_bins, int n_bins,
+                              const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s);
 hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, const FbpGeom& g, int batch,
                              hipStream_t s);
 hipError_t launch_fbp_backproject(const float* filt, float* img, const float2* cs, const FbpGeom& g, int batch,

@@ -13,6 +13,8 @@ struct ske_fbp {
     ske::FbpGeom g{};
     int max_batch = 0;
     ske::DevBuf<float2> cs;
+    ske::DevBuf<int> ang;  // angle indices: those with |cos| >= |sin| first (n_bins of them), then the rest
+    int n_bins = 0;
     ske::DevBuf<float> taps;
     std::vector<float> taps_host;
     ske::DevBuf<float> buf[4];  // image, sinogram, filtered, reconstruction

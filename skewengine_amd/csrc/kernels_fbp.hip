@@ -8,7 +8,9 @@
 //
 //   project     p(a,k) = delta * sum_t f(s_k cos - t sin, s_k sin + t cos),
 //               t on a delta grid over the chord of the ray through the
-//               image square, f bilinear (zero outside);
+//               image square, f bilinear (zero outside); lanes run along
+//               whichever of bins or samples keeps the gathers on one image
+//               row;
 //   filter      q(a,k) = sum_k' p(a,k') h(k-k'), h = delta * Ram-Lak
 //               (h0 = 1/(4 delta^2), odd m: -1/(m^2 pi^2 delta^2), even: 0) or
 //               learned taps;
@@ -43,20 +45,12 @@ __device__ __forceinline__ float bilinear(const float* __restrict__ img, int n, 
     return top + wv * (bot - top);
 }
 
-// One thread per (bin, angle, slice).
-__global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ img, float* __restrict__ sino,
-                                                     const float2* __restrict__ cs, FbpGeom g) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
-    const int b = blockIdx.z;
-    if (k >= g.ndet) return;
-    const float2 t2 = cs[a];
-    const float c = t2.x, sn = t2.y;
-    const float s = ((float)k - g.det_c) * g.delta;
-    // The bilinear image is zero farther than 1 + delta/2 from the centre
-    // along either axis, so samples outside the chord of the line
-    // x c + y sn = s through the square [-B, B]^2, B = 1 + delta, add exact
-    // zeros and are skipped (x = s c - t sn, y = s sn + t c).
+// Sample range [i0, i1] of the ray of bin k at angle (c, sn): the samples
+// t_i = (i - (nt-1)/2) delta inside the chord of the line x c + y sn = s
+// through the square [-B, B]^2, B = 1 + delta (the bilinear image is zero
+// farther than 1 + delta/2 from the centre along either axis, so samples
+// outside add exact zeros).  x = s c - t sn, y = s sn + t c.  Empty: i1 < i0.
+__device__ __forceinline__ void ray_range(const FbpGeom& g, float s, float c, float sn, int& i0, int& i1) {
     const float B = 1.0f + g.delta;
     float tlo = -g.tmax, thi = g.tmax;
     if (fabsf(sn) > 1e-7f) {
@@ -73,24 +67,67 @@ __global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ i
     } else if (fabsf(s * sn) > B) {
         thi = tlo - 1.0f;
     }
+    i0 = 0;
+    i1 = -1;
+    if (thi >= tlo) {
+        const float half = 0.5f * (float)(g.nt - 1);
+        i0 = max((int)ceilf(tlo / g.delta + half) - 1, 0);
+        i1 = min((int)floorf(thi / g.delta + half) + 1, g.nt - 1);
+    }
+}
+
+__device__ __forceinline__ float ray_sample(const float* __restrict__ im, const FbpGeom& g, float s, float c,
+                                            float sn, int i) {
+    const float half = 0.5f * (float)(g.nt - 1);
+    const float inv = 1.0f / g.delta;
+    const float t = ((float)i - half) * g.delta;
+    const float x = s * c - t * sn;
+    const float y = s * sn + t * c;
+    return bilinear(im, g.n, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
+}
+
+// Angles with |cos| >= |sin|: one thread per (bin, angle, slice) summing its
+// ray's samples in order.  Neighbouring lanes (bins) sample points
+// delta (cos, sin) apart: nearly the same image row, so the gathers
+// coalesce.  `ang` lists the angle indices this launch covers.
+__global__ __launch_bounds__(256) void k_fbp_project_bins(const float* __restrict__ img, float* __restrict__ sino,
+                                                          const float2* __restrict__ cs, const int* __restrict__ ang,
+                                                          FbpGeom g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = ang[blockIdx.y];
+    const int b = blockIdx.z;
+    if (k >= g.ndet) return;
+    const float2 t2 = cs[a];
+    const float s = ((float)k - g.det_c) * g.delta;
+    int i0, i1;
+    ray_range(g, s, t2.x, t2.y, i0, i1);
     const float* im = img + (size_t)b * g.n * g.n;
     float acc = 0.0f;
-    if (thi >= tlo) {
-        // samples t_i = (i - (nt-1)/2) * delta, i in [i0, i1] covering [tlo, thi]
-        const float half = 0.5f * (float)(g.nt - 1);
-        int i0 = (int)ceilf(tlo / g.delta + half) - 1;
-        int i1 = (int)floorf(thi / g.delta + half) + 1;
-        i0 = max(i0, 0);
-        i1 = min(i1, g.nt - 1);
-        const float inv = 1.0f / g.delta;
-        for (int i = i0; i <= i1; ++i) {
-            const float t = ((float)i - half) * g.delta;
-            const float x = s * c - t * sn;
-            const float y = s * sn + t * c;
-            acc += bilinear(im, g.n, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
-        }
-    }
+    for (int i = i0; i <= i1; ++i) acc += ray_sample(im, g, s, t2.x, t2.y, i);
     sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
+}
+
+// Angles with |sin| > |cos|: one wave per ray, lanes over its samples
+// (neighbouring samples are delta (-sin, cos) apart: again nearly one image
+// row), then a wave reduction.
+__global__ __launch_bounds__(256) void k_fbp_project_rays(const float* __restrict__ img, float* __restrict__ sino,
+                                                          const float2* __restrict__ cs, const int* __restrict__ ang,
+                                                          FbpGeom g) {
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int a = ang[blockIdx.y];
+    const int b = blockIdx.z;
+    if (k >= g.ndet) return;
+    const float2 t2 = cs[a];
+    const float s = ((float)k - g.det_c) * g.delta;
+    int i0, i1;
+    ray_range(g, s, t2.x, t2.y, i0, i1);
+    const float* im = img + (size_t)b * g.n * g.n;
+    float acc = 0.0f;
+    for (int i = i0 + lane; i <= i1; i += 64) acc += ray_sample(im, g, s, t2.x, t2.y, i);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
 }
 
 // One workgroup per sinogram row: the row and the 2*ndet-1 filter taps in
@@ -194,10 +231,14 @@ __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict
     if (in) img[(size_t)b * g.n * g.n + (size_t)i * g.n + j] = acc * scale;
 }
 
-hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const FbpGeom& g, int batch,
-                              hipStream_t s) {
-    dim3 grid((g.ndet + 255) / 256, g.nang, batch);
-    hipLaunchKernelGGL(k_fbp_project, grid, dim3(256), 0, s, img, sino, cs, g);
+hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const int* ang_bins, int n_bins,
+                              const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s) {
+    if (n_bins > 0)
+        hipLaunchKernelGGL(k_fbp_project_bins, dim3((g.ndet + 255) / 256, n_bins, batch), dim3(256), 0, s, img, sino,
+                           cs, ang_bins, g);
+    if (n_rays > 0)
+        hipLaunchKernelGGL(k_fbp_project_rays, dim3((g.ndet + 3) / 4, n_rays, batch), dim3(256), 0, s, img, sino, cs,
+                           ang_rays, g);
     return hipGetLastError();
 }
 
