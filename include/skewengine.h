@@ -105,6 +105,10 @@ int ske_get_params(ske_engine *e, ske_sector_params *p);
 int ske_get_tables(ske_engine *e, int *target, float *weight, unsigned short *first, unsigned short *last);
 /* skewInput / skewOutput rows (skewHeight*dim_i floats each; either may be NULL). */
 int ske_get_skew(ske_engine *e, float *skewInput, float *skewOutput);
+/* Host-side operators (the kernelcpu form, src/skewEngine.cuh:370,696): after
+ * ske_get_skew, the caller's function fills skewOutput on the host and this
+ * uploads its skewHeight*dim_i floats as the current sector's skewOutput. */
+int ske_set_skew_output(ske_engine *e, const float *skewOutput);
 /* Device pointers of the current sector's buffers (valid until the next skew()). */
 int ske_device_pointers(ske_engine *e, float **d_skewInput, float **d_skewOutput, unsigned short **d_first,
                         unsigned short **d_last);

@@ -59,5 +59,21 @@ def build(verbose=False):
     return LIB
 
 
+def build_facade_check(verbose=False):
+    """Compile tests/facade_check.hip, a program written against the C++
+    facade include/skewEngine.hpp, into tests/_build/libfacade_check.so
+    (loaded by the GPU tests; needs libskewengine.so first)."""
+    tests = os.path.join(ROOT, "tests")
+    out = os.path.join(tests, "_build", "libfacade_check.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    _run([HIPCC, "-x", "hip", os.path.join(tests, "facade_check.hip"), "-o", out + ".tmp", "-shared", "-fPIC", "-O2",
+          "-std=c++17", "-I" + INCLUDE] + DEVICE + ["-L" + HERE, "-lskewengine", "-Wl,-rpath,$ORIGIN/../../skewengine_amd"])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_facade_check(verbose=True)

@@ -819,6 +819,15 @@ int ske_get_skew(ske_engine* e, float* skewInput, float* skewOutput) {
     return c.sync();
 }
 
+int ske_set_skew_output(ske_engine* e, const float* skewOutput) {
+    if (!e || !skewOutput) return fail(SKE_EINVAL, "engine or skewOutput is NULL");
+    if (!e->have_sector) return fail(SKE_ESTATE, "no sector skewed yet");
+    Core& c = e->core;
+    const size_t cells = (size_t)e->cur.H * e->cur.dim_i;
+    SKE_HIP(hipMemcpyAsync(e->SO.p, skewOutput, cells * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    return c.sync();
+}
+
 int ske_device_pointers(ske_engine* e, float** d_skewInput, float** d_skewOutput, unsigned short** d_first,
                         unsigned short** d_last) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
