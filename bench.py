@@ -86,7 +86,10 @@ def bench_viewshed_batch(args, ske, synth):
     sides, hursts = synth.viewshed_batch_sizes(32, seed=4)
     dems = [synth.dem(n, h, 1000 + i, 3000.0) for i, (n, h) in enumerate(zip(sides, hursts))]
     _progress("synthesised %d DEMs" % len(dems))
-    batch = ske.SkewBatch(dems)  # inputs now resident in HBM
+    t_in = time.perf_counter()
+    batch = ske.SkewBatch(dems)  # inputs now resident in HBM (host -> device copy + prepare)
+    batch.synchronize()
+    t_in = time.perf_counter() - t_in
     del dems
     steps_total = 0.0
     pts_sectors = 0.0
@@ -110,6 +113,9 @@ def bench_viewshed_batch(args, ske, synth):
     t1 = time.perf_counter()
     _progress("timed %d step(s): %.2f s" % (args.steps, t1 - t0))
     sec_per_step = (t1 - t0) / args.steps
+    t_out = time.perf_counter()
+    batch.outputs()  # device -> host copy of the 32 result maps
+    t_out = time.perf_counter() - t_out
     k_ms, k_cnt = batch.profile("op_sdem")
     achieved = OPS_PER_STEP * steps_total * args.steps / (k_ms / 1e3) / 1e12
     result = {
@@ -121,7 +127,9 @@ def bench_viewshed_batch(args, ske, synth):
         "config": {"workload": "batched total viewshed: 32 fBm DEMs, sides %s, Hurst U[0.7,0.9], heights 0-3000 m, "
                                "10 m cells, observer 1.5 m, 180 sectors x 2 directions" % sorted(set(sides)),
                    "dems": len(sides), "sides": sides, "ray_steps_per_step": steps_total,
-                   "pts_sectors_per_step": pts_sectors, "seed": 4},
+                   "pts_sectors_per_step": pts_sectors, "seed": 4,
+                   "end_to_end_s_incl_host_copies": t_in + sec_per_step + t_out,
+                   "host_to_device_and_prepare_s": t_in, "device_to_host_s": t_out},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_VALU_TOPS, "traffic": _traffic("op_sdem"),
                      "kernel": "op_sdem", "kernel_ms_total": k_ms, "launches": k_cnt,
