@@ -231,7 +231,9 @@ struct Core {
         SKE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         std::vector<float> r;
         recip_table(kMaxDim + 1, r);
-        SKE_TRY(rtab.upload(reinterpret_cast<const float4*>(r.data()), kMaxDim + 1, stream));
+        // float4 table of kMaxDim+1 entries, then the float2 pairs (read as
+        // float4 storage: 2*(kMaxDim+33) floats, an even number of float4s)
+        SKE_TRY(rtab.upload(reinterpret_cast<const float4*>(r.data()), r.size() / 4, stream));
         std::vector<double> t;
         std::vector<int> off;
         fft_twiddles(kMaxFftLog, t, off);

@@ -140,16 +140,21 @@ __device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) 
 // The visibility flag is the sign of t = M - q: rounding preserves the sign
 // of the exact difference and M == q gives +0, so sign(t) = (q > M).
 // alignbit(w, t, 31) = (w << 1) | (t >> 31) appends it in one operation.
+// The 32 reciprocal pairs of a block are loaded up front (wave-uniform:
+// scalar loads, one wait), so the steps themselves wait only on LDS.
 template <int kDir, bool kTail>
 __device__ __forceinline__ uint32_t scan_block(const float* rowj, float h, const float4* __restrict__ rtab, int d0,
                                                int myLen, float& M) {
+    const float2* __restrict__ ytab = reinterpret_cast<const float2*>(rtab + kMaxDim + 1);
+    float2 y[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) y[u] = ytab[d0 + 1 + u];
     uint32_t w = 0;
 #pragma unroll
     for (int u = 1; u <= 32; ++u) {
         const int d = d0 + u;
-        const float4 y = rtab[d];
         const float a = rowj[kDir * d] - h;
-        const float q = __builtin_fmaf(a, y.x, a * y.y);
+        const float q = __builtin_fmaf(a, y[u - 1].x, a * y[u - 1].y);
         float t = M - q;
         if (kTail) t = (d > myLen) ? -1.0f : t;  // past this lane's row end: record visible
         M = fmaxf(M, q);

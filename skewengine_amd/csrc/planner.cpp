@@ -149,17 +149,23 @@ void plan_append(int dimx, int dimy, int s0, int s1, long long img, ImagePlan& i
 }
 
 void recip_table(int n, std::vector<float>& tab4) {
-    tab4.assign(4 * (size_t)n, 0.0f);
-    for (int d = 1; d < n; d++) {
+    tab4.assign(4 * (size_t)n + 2 * ((size_t)n + 32), 0.0f);
+    float* tab2 = tab4.data() + 4 * (size_t)n;
+    for (int d = 1; d < n + 32; d++) {
         const float yh = (float)(1.0 / d);
         // 1/d - yh: d*yh is exact in double and 1 - d*yh is exact (Sterbenz),
         // so the only roundings are the final double division and the float.
         const double res = (1.0 - (double)d * (double)yh) / d;
-        tab4[4 * d + 0] = yh;
-        tab4[4 * d + 1] = (float)res;
-        tab4[4 * d + 2] = (float)d;
-        tab4[4 * d + 3] = (float)((double)d * d);
+        if (d < n) {
+            tab4[4 * d + 0] = yh;
+            tab4[4 * d + 1] = (float)res;
+            tab4[4 * d + 2] = (float)d;
+            tab4[4 * d + 3] = (float)((double)d * d);
+        }
+        tab2[2 * d + 0] = yh;
+        tab2[2 * d + 1] = (float)res;
     }
+    tab4.resize((tab4.size() + 3) / 4 * 4, 0.0f);  // whole float4s
 }
 
 void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off) {

@@ -59,7 +59,9 @@ void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_
 void plan_append(int dimx, int dimy, int s0, int s1, long long img, ImagePlan& ip, int range8[8]);
 void plan_finish(ImagePlan& ip, int order);
 
-// {RN(1/d), RN(1/d - RN(1/d)), d, d*d} for d = 0..n-1.
+// {RN(1/d), RN(1/d - RN(1/d)), d, d*d} for d = 0..n-1, followed by the
+// pairs {RN(1/d), RN(1/d - RN(1/d))} for d = 0..n+31 (the viewshed's
+// block-preloaded table).
 void recip_table(int n, std::vector<float>& tab4);
 
 // Twiddles of the reference FFT (src/helper.h:63-81) for N = 2^m, m = 1..mmax,
