@@ -194,8 +194,9 @@ __global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict
 // conv 32 -> 1 + bias, residual add; optionally the loss and its gradient.
 // The a3 halo stays bf16 in LDS with a 17-word pixel pitch, so the lanes
 // (consecutive pixels) read distinct banks.  Channel pairs are summed with
-// v_dot2c_f32_bf16 (weights rounded to bf16), taps in order, channels
-// 0-1, 2-3, ... within a tap.
+// v_dot2c_f32_bf16 (weights rounded to bf16) in two partial sums, channels
+// 0-15 and 16-31 (taps in order, pairs in order within a tap), then
+// (s0 + s1) + b4.
 constexpr int kOP = C / 2 + 1;  // LDS pixel pitch in 32-bit words
 __device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
@@ -226,14 +227,16 @@ __global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict
     }
     __syncthreads();
     const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float s = b4;
+    float s0 = 0.0f, s1 = 0.0f;  // channels 0-15 and 16-31
     for (int tap = 0; tap < 9; tap++) {
         const uint32_t* xp = xs + ((ty + tap / 3) * (T + 2) + tx + tap % 3) * kOP;
 #pragma unroll
-        for (int c2 = 0; c2 < C / 2; c2++) s = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s);
+        for (int c2 = 0; c2 < C / 4; c2++) s0 = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s0);
+#pragma unroll
+        for (int c2 = C / 4; c2 < C / 2; c2++) s1 = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s1);
     }
     const size_t p = ((size_t)b * n + i0 + ty) * n + j0 + tx;
-    const float o = r0[p] + s;
+    const float o = r0[p] + ((s0 + s1) + b4);
     out[p] = o;
     if (target) {
         const float d = o - target[p];
@@ -278,15 +281,17 @@ __device__ __forceinline__ void fused_acc_init(f32x16& acc, const float* bias, i
     }
 }
 
-__device__ __forceinline__ void fused_mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
-                                                const uint16_t* __restrict__ wb, const float* bias, int wave, int l32,
-                                                int hi) {
-    const int wo = wi - 2, count = wo * wo;
-    bf16x8 w[18];
+__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[18], const uint16_t* __restrict__ wb, int l32, int hi) {
 #pragma unroll
     for (int t = 0; t < 9; t++)
 #pragma unroll
         for (int h = 0; h < 2; h++) w[2 * t + h] = as_bf16x8(*(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi));
+}
+
+__device__ __forceinline__ void fused_mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
+                                                const bf16x8 (&w)[18], const float* bias, int wave, int l32,
+                                                int hi) {
+    const int wo = wi - 2, count = wo * wo;
     for (int g0 = wave; g0 * 32 < count; g0 += 16) {
         const int g1 = g0 + 8;
         const int p0 = g0 * 32 + l32, p1 = g1 * 32 + l32;
@@ -341,20 +346,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    // all global loads of the prologue in flight together: W2 operands, W1,
+    // biases, W4 and the r0 tile
+    bf16x8 w[18];
+    load_mid_weights(w, wb2, l32, hi);
+    const bf16x8 w1 = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
+    float bias1[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) bias1[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
     for (int t = threadIdx.x; t < kF0 * kF0; t += 512) {
         const int ii = i0 - 4 + t / kF0, jj = j0 - 4 + t % kF0;
         s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
     }
     for (int t = threadIdx.x; t < 9 * C / 2; t += 512) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
     if (threadIdx.x < 2 * C) sbias[threadIdx.x] = prm[(threadIdx.x < C ? kB2 : kB3 - C) + threadIdx.x];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l32 = lane & 31, hi = lane >> 5;
     __syncthreads();
     {   // a1 on the 22 x 22 region at (i0-3, j0-3)
-        const bf16x8 w = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
-        float bias[16];
-#pragma unroll
-        for (int r = 0; r < 16; r++) bias[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
         const int count = kF1 * kF1;
         for (int g = wave; g * 32 < count; g += 8) {
             const int p = g * 32 + l32;
@@ -362,34 +371,40 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             const int y = pc / kF1, x = pc % kF1;
             f32x16 acc;
 #pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = bias[r];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, conv_in_operand(s0, kF0, y, x, hi), acc, 0, 0, 0);
+            for (int r = 0; r < 16; r++) acc[r] = bias1[r];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, conv_in_operand(s0, kF0, y, x, hi), acc, 0, 0, 0);
             const int iy = i0 - 3 + y, ix = j0 - 3 + x;
             if (p < count) store_group(sa + p * kMP + 4 * hi, acc, iy >= 0 && iy < n && ix >= 0 && ix < n);
         }
     }
     __syncthreads();
-    fused_mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, wb2, sbias, wave, l32, hi);
+    fused_mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, w, sbias, wave, l32, hi);
+    load_mid_weights(w, wb3, l32, hi);  // in flight while the other waves finish
     __syncthreads();
-    fused_mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, wb3, sbias + C, wave, l32, hi);
+    fused_mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, w, sbias + C, wave, l32, hi);
     __syncthreads();
-    if (threadIdx.x < kFT * kFT) {  // conv 32 -> 1 + bias + residual, the order of k_conv_out_fwd
-        const int y = threadIdx.x / kFT, x = threadIdx.x % kFT;
-        float acc = prm[kB4];
+    {   // conv 32 -> 1 + bias + residual: two threads per pixel (channels
+        // 0-15 and 16-31), combined as in k_conv_out_fwd
+        const int px = threadIdx.x >> 1, part = threadIdx.x & 1;
+        const int y = px / kFT, x = px % kFT;
+        float acc = 0.0f;
         for (int tap = 0; tap < 9; tap++) {
-            const uint16_t* xp = sa + ((y + tap / 3) * kF3 + x + tap % 3) * kMP;
+            const uint16_t* xp = sa + ((y + tap / 3) * kF3 + x + tap % 3) * kMP + 16 * part;
 #pragma unroll
-            for (int c8 = 0; c8 < 4; c8++) {
+            for (int c8 = 0; c8 < 2; c8++) {
                 const uint4 v = *(const uint4*)(xp + 8 * c8);
-                const uint32_t* wp = w4 + tap * (C / 2) + 4 * c8;
+                const uint32_t* wp = w4 + tap * (C / 2) + 8 * part + 4 * c8;
                 acc = dot2_bf16(wp[0], v.x, acc);
                 acc = dot2_bf16(wp[1], v.y, acc);
                 acc = dot2_bf16(wp[2], v.z, acc);
                 acc = dot2_bf16(wp[3], v.w, acc);
             }
         }
-        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
-        out[p] = img[(size_t)(i0 + y) * n + j0 + x] + acc;
+        const float other = __shfl_xor(acc, 1, 64);
+        if (part == 0) {
+            const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
+            out[p] = img[(size_t)(i0 + y) * n + j0 + x] + ((acc + other) + prm[kB4]);
+        }
     }
 }
 
