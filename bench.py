@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 # of v_sub/v_max, consistent with that figure.
 PEAK_VALU_TOPS = 157.3 / 2
 OPS_PER_STEP = 4  # sub, divide, compare, max per ray step (src/kernelSDEM.h:191-196)
+# VALU instructions per executed lane-step of the bit-block scan loop in the
+# gfx950 build: 246 per 32-step block (6 per step, then the popcount flush,
+# the early-exit bound and the loop control).
+VALU_PER_LANE_STEP = 246 / 32
 HBM_PEAK_GBS = 8000.0
 
 
@@ -118,6 +122,15 @@ def bench_viewshed_batch(args, ske, synth):
     t_out = time.perf_counter() - t_out
     k_ms, k_cnt = batch.profile("op_sdem")
     achieved = OPS_PER_STEP * steps_total * args.steps / (k_ms / 1e3) / 1e12
+    # One more (untimed) step with the kernels counting the ray steps their
+    # lanes execute: the scans stop once no later target can be visible.
+    batch.set_option(_lib.OPT_COUNT_STEPS, 1)
+    batch.profile_reset()
+    batch.run(ske.KERNEL_SDEM, 0, 180, ske.POVH_DEFAULT)
+    lane_steps = batch.counter("op_sdem_lane_steps")
+    batch.set_option(_lib.OPT_COUNT_STEPS, 0)
+    _progress("counted %.4e executed lane-steps (%.3f of the reference's ray steps)"
+              % (lane_steps, lane_steps / steps_total))
     result = {
         "metric": "Mpts*sectors/s (batched total viewshed)",
         "value": pts_sectors / 1e6 / sec_per_step,
@@ -134,6 +147,10 @@ def bench_viewshed_batch(args, ske, synth):
                      "frac": achieved / PEAK_VALU_TOPS, "traffic": _traffic("op_sdem"),
                      "kernel": "op_sdem", "kernel_ms_total": k_ms, "launches": k_cnt,
                      "ray_steps_per_s": steps_total * args.steps / (k_ms / 1e3),
+                     "executed_lane_steps_per_step": lane_steps,
+                     "executed_over_reference_steps": lane_steps / steps_total,
+                     "issued_valu_frac": VALU_PER_LANE_STEP * lane_steps * args.steps / (k_ms / 1e3) / 1e12
+                                         / PEAK_VALU_TOPS,
                      "autotuned": {"waves_per_workgroup": waves, "persistent_workgroups_per_cu": pers,
                                    "waves_per_simd_hint": occ}},
     }

@@ -135,10 +135,16 @@ int ske_synchronize(ske_engine *e);
 #define SKE_OPT_SDEM_OCCUPANCY 7  /* viewshed persistent grid: 0 default, 8 compiled for 8 waves/SIMD */
 #define SKE_OPT_RADON_LOCAL 8     /* 1: SKE_KERNEL_RADON computes the local transform (pointRadon per
                                    * cell), the reference's LRT=true (src/kernelRADN.h:212-220) */
+#define SKE_OPT_COUNT_STEPS 9     /* 1: the all-sector bit-block viewshed counts the ray steps its
+                                   * lanes execute (counter "op_sdem_lane_steps"; the scans stop
+                                   * early once no later target can be visible) */
 int ske_set_option(ske_engine *e, int option, long long value);
 /* Accumulated HIP-event time of the launches named `name` since the last
  * reset (profiling option on): total milliseconds and launch count. */
 int ske_profile_get(ske_engine *e, const char *name, double *total_ms, int *count);
+/* Accumulated value of a work counter since the last profile reset
+ * ("op_sdem_lane_steps" with SKE_OPT_COUNT_STEPS on); 0 if never counted. */
+int ske_counter_get(ske_engine *e, const char *name, double *value);
 int ske_profile_reset(ske_engine *e);
 
 /* Work counts of the exact maps for sectors [s0,s1) of a dimx x dimy grid:
@@ -166,6 +172,7 @@ int ske_batch_output_device(ske_batch *b, int index, const float **d_output);
 int ske_batch_synchronize(ske_batch *b);
 int ske_batch_set_option(ske_batch *b, int option, long long value);
 int ske_batch_profile_get(ske_batch *b, const char *name, double *total_ms, int *count);
+int ske_batch_counter_get(ske_batch *b, const char *name, double *value);
 int ske_batch_profile_reset(ske_batch *b);
 /* Autotuner of the viewshed launch shape: times candidate (waves per
  * workgroup, persistent workgroups per CU, occupancy hint) triples on a

@@ -14,6 +14,7 @@ KERNEL_SDEM, KERNEL_IDENTITYDEM, KERNEL_RADON, KERNEL_IDENTITYIMG, KERNEL_CEPSTR
 OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE, OPT_GRAPH, OPT_SDEM_WAVES, OPT_SDEM_PERSISTENT = 1, 2, 3, 4, 5, 6
 OPT_SDEM_OCCUPANCY = 7
 OPT_RADON_LOCAL = 8
+OPT_COUNT_STEPS = 9
 LEARNED_TAPS = 19105
 LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180
@@ -24,10 +25,10 @@ EXPORTS = [
     "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
     "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_get_skew", "ske_set_skew_output",
     "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option",
-    "ske_profile_get", "ske_profile_reset", "ske_work",
+    "ske_profile_get", "ske_counter_get", "ske_profile_reset", "ske_work",
     "ske_batch_create", "ske_batch_create_device", "ske_batch_destroy", "ske_batch_run", "ske_batch_output",
     "ske_batch_output_device", "ske_batch_synchronize", "ske_batch_set_option", "ske_batch_profile_get",
-    "ske_batch_profile_reset", "ske_batch_autotune",
+    "ske_batch_counter_get", "ske_batch_profile_reset", "ske_batch_autotune",
     "ske_fbp_create", "ske_fbp_destroy", "ske_fbp_upload", "ske_fbp_download", "ske_fbp_buffer", "ske_fbp_project",
     "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct", "ske_fbp_set_taps", "ske_fbp_get_taps",
     "ske_fbp_synchronize", "ske_fbp_set_option", "ske_fbp_profile_get", "ske_fbp_profile_reset",
@@ -95,6 +96,7 @@ def lib():
     L.ske_synchronize.argtypes = [vp]
     L.ske_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
     L.ske_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
+    L.ske_counter_get.argtypes = [vp, ctypes.c_char_p, f64p]
     L.ske_profile_reset.argtypes = [vp]
     L.ske_work.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p, f64p]
     L.ske_batch_create.argtypes = [ctypes.c_int, i32p, P(vp), ctypes.c_int, P(vp)]
@@ -106,6 +108,7 @@ def lib():
     L.ske_batch_synchronize.argtypes = [vp]
     L.ske_batch_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
     L.ske_batch_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
+    L.ske_batch_counter_get.argtypes = [vp, ctypes.c_char_p, f64p]
     L.ske_batch_profile_reset.argtypes = [vp]
     L.ske_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_float, i32p, i32p, i32p, f64p]
     L.ske_fbp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, P(vp)]

@@ -143,6 +143,8 @@ struct Core {
     long long arena_bytes = 48LL << 30;
     bool profile = false;
     bool use_graph = false;
+    bool count_steps = false;  // viewshed: count executed lane-steps (counter[2..3] as a u64)
+    std::map<std::string, double> counters;
     // profiling
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
     std::vector<hipEvent_t> event_pool;
@@ -206,8 +208,16 @@ struct Core {
         return SKE_OK;
     }
 
+    unsigned long long* step_counter() { return reinterpret_cast<unsigned long long*>(counter.p + 2); }
+
     int sync() {
         SKE_HIP(hipStreamSynchronize(stream));
+        if (count_steps) {
+            unsigned long long v = 0;
+            SKE_HIP(hipMemcpy(&v, step_counter(), sizeof(v), hipMemcpyDeviceToHost));
+            SKE_HIP(hipMemset(step_counter(), 0, sizeof(v)));
+            counters["op_sdem_lane_steps"] += (double)v;
+        }
         for (auto& t : pending) {
             float ms = 0.0f;
             SKE_HIP(hipEventElapsedTime(&ms, std::get<1>(t), std::get<2>(t)));
@@ -246,6 +256,7 @@ struct Core {
         lrt_gauss_table(gt);
         SKE_TRY(gtab.upload(gt.data(), gt.size(), stream));
         SKE_TRY(counter.ensure(16));
+        SKE_HIP(hipMemsetAsync(counter.p, 0, 16 * sizeof(int), stream));
         SKE_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
         return SKE_OK;
     }
@@ -388,7 +399,8 @@ struct Core {
             case SKE_KERNEL_SDEM:
                 if (sdem_variant == 0) {
                     SKE_TRY(launch("op_sdem", [&] {
-                        return launch_op_sdem_tuned(v, in4.p, nullptr, arena.p, rtab.p, povh, tune, counter.p, stream);
+                        return launch_op_sdem_tuned(v, in4.p, nullptr, arena.p, rtab.p, povh, tune, counter.p,
+                                                    count_steps ? step_counter() : nullptr, stream);
                     }));
                 } else {
                     SKE_TRY(launch("op_sdem", [&] {
@@ -501,9 +513,21 @@ struct Core {
             radon_local = value != 0;
             drop_graph();
             return SKE_OK;
+        case SKE_OPT_COUNT_STEPS:
+            SKE_TRY(sync());  // earlier launches are counted (or not) as they were issued
+            count_steps = value != 0;
+            drop_graph();
+            return SKE_OK;
         default:
             return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
         }
+    }
+
+    int counter_get(const char* name, double* value) {
+        SKE_TRY(sync());
+        auto it = counters.find(name);
+        if (value) *value = it == counters.end() ? 0.0 : it->second;
+        return SKE_OK;
     }
 
     int profile_get(const char* name, double* total_ms, int* count) {
@@ -537,9 +561,9 @@ struct Core {
         double best = 1e300;
         SdemTuning pick = tune;
         for (const SdemTuning& c : cands) {
-            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, stream));
+            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, nullptr, stream));
             SKE_HIP(hipEventRecord(a, stream));
-            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, stream));
+            SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, nullptr, stream));
             SKE_HIP(hipEventRecord(b, stream));
             SKE_HIP(hipEventSynchronize(b));
             float ms = 0.0f;
@@ -880,7 +904,13 @@ int ske_profile_reset(ske_engine* e) {
     if (!e) return fail(SKE_EINVAL, "engine is NULL");
     SKE_TRY(e->core.sync());
     e->core.prof.clear();
+    e->core.counters.clear();
     return SKE_OK;
+}
+
+int ske_counter_get(ske_engine* e, const char* name, double* value) {
+    if (!e || !name) return fail(SKE_EINVAL, "engine or name is NULL");
+    return e->core.counter_get(name, value);
 }
 
 int ske_work(int dimx, int dimy, int s0, int s1, double* cells, double* steps) {
@@ -982,7 +1012,13 @@ int ske_batch_profile_reset(ske_batch* b) {
     if (!b) return fail(SKE_EINVAL, "batch is NULL");
     SKE_TRY(b->core.sync());
     b->core.prof.clear();
+    b->core.counters.clear();
     return SKE_OK;
+}
+
+int ske_batch_counter_get(ske_batch* b, const char* name, double* value) {
+    if (!b || !name) return fail(SKE_EINVAL, "batch or name is NULL");
+    return b->core.counter_get(name, value);
 }
 
 int ske_batch_autotune(ske_batch* b, int kernel_id, float povh, int* waves, int* wgs_per_cu, int* occupancy,

@@ -128,6 +128,10 @@ __global__ __launch_bounds__(256) void k_op_sdem(const SecDesc* __restrict__ sec
 // there) is reproduced bit for bit.
 // ---------------------------------------------------------------------------
 constexpr int kPadBits = 128;
+constexpr int kChunksMax = 128;  // 32-cell chunks of a row of <= 4096 cells
+
+// LDS of the bit-block kernels: guard band, row, guard band, chunk maxima.
+inline size_t sdem_bits_lds(int maxL) { return sizeof(float) * ((size_t)maxL + 2 * kPadBits + 2 * kChunksMax); }
 
 __device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) {
     const int pc = __popc(w);
@@ -163,15 +167,89 @@ __device__ __forceinline__ uint32_t scan_block(const float* rowj, float h, const
     return w;
 }
 
+// Early exit.  After the steps up to db, no later target k of this lane can
+// be visible once an upper bound of its quotient is <= M: with a = Smax - h
+// (Smax the largest height among the remaining targets, taken from the
+// per-32-cell maxima `cmax`), every later a_k <= a by monotone rounding, so
+// q_k <= RN(a / (db+1)) for a >= 0 and q_k <= RN(a / myLen) for a < 0.  The
+// bound is evaluated with one-sided slack (2^-20 relative), which keeps it
+// above those rounded quotients.  When every lane of the wave is past its row
+// end or bounded, the scan stops: a lane with steps left has only invisible
+// steps ahead, so its trailing visible run closes and counts (cv = A); a
+// lane whose row ended keeps the usual correction of the padded steps.
+template <int kDir>
+__device__ __forceinline__ bool scan_done(const float* __restrict__ cmax, int jj, int L, float h, float rl,
+                                          const float2* __restrict__ ytab, int myLen, int db, float M) {
+    int pos = jj + kDir * (db + 1);
+    pos = pos < 0 ? 0 : (pos > L - 1 ? L - 1 : pos);  // lanes with myLen <= db are done anyway
+    const float a = cmax[pos >> 5] - h;
+    const float bpos = (a * ytab[db + 1].x) * 1.00000095367431640625f;
+    const float bneg = (a * rl) * 0.99999904632568359375f;
+    const bool done = (myLen <= db) | ((a >= 0.0f ? bpos : bneg) <= M);
+    // every lane of the wave is active here; a ballot keeps the exit uniform
+    return __builtin_amdgcn_readfirstlane((int)(__builtin_amdgcn_ballot_w64(!done) != 0)) == 0;
+}
+
 template <int kDir>
 __device__ __forceinline__ int scan_dir(const float* rowj, float h, const float4* __restrict__ rtab, int myLen,
-                                        int dfull, int D) {
+                                        int dfull, int D, const float* __restrict__ cmax, int jj, int L,
+                                        int& nblk) {
+    const float2* __restrict__ ytab = reinterpret_cast<const float2*>(rtab + kMaxDim + 1);
+    const float rl = 1.0f / (float)(myLen > 0 ? myLen : 1);
     float M = -2000.0f;
     int A = 1, s = 0;
     int d0 = 0;
-    for (; d0 + 32 <= dfull; d0 += 32) flush_block(scan_block<kDir, false>(rowj, h, rtab, d0, myLen, M), d0, A, s);
-    for (; d0 < D; d0 += 32) flush_block(scan_block<kDir, true>(rowj, h, rtab, d0, myLen, M), d0, A, s);
-    return A - ((D + 1) * (D + 1) - s * s);
+    int e1 = dfull, e2 = D;  // loop ends, lowered to the current step when the scan stops
+    for (; d0 + 32 <= e1; d0 += 32) {
+        flush_block(scan_block<kDir, false>(rowj, h, rtab, d0, myLen, M), d0, A, s);
+        if (scan_done<kDir>(cmax, jj, L, h, rl, ytab, myLen, d0 + 32, M)) e1 = e2 = d0 + 32;
+    }
+    for (; d0 < e2; d0 += 32) {
+        flush_block(scan_block<kDir, true>(rowj, h, rtab, d0, myLen, M), d0, A, s);
+        if (scan_done<kDir>(cmax, jj, L, h, rl, ytab, myLen, d0 + 32, M)) e2 = d0 + 32;
+    }
+    nblk += d0 >> 5;
+    return myLen > d0 ? A : A - ((d0 + 1) * (d0 + 1) - s * s);
+}
+
+// Per-32-cell maxima of the staged row (cm[c] = max row[32c .. 32c+31]),
+// then their suffix maxima (suf) and prefix maxima (pre), for the early exit
+// of the right and left scans.  Rows of at most 4096 cells: <= 128 chunks.
+template <int kWaves>
+__device__ __forceinline__ void row_chunk_max(const float* row, int L, float* suf, float* pre) {
+    const int nch = (L + 31) >> 5;
+    for (int t0 = 0; t0 < nch * 32; t0 += 64 * kWaves) {
+        const int t = t0 + (int)threadIdx.x;
+        float v = t < L ? row[t] : -INFINITY;
+#pragma unroll
+        for (int o = 16; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if ((t & 31) == 0 && t < nch * 32) suf[t >> 5] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // one wave: two chunks per lane
+        const int l = threadIdx.x;
+        const float x0 = 2 * l < nch ? suf[2 * l] : -INFINITY;
+        const float x1 = 2 * l + 1 < nch ? suf[2 * l + 1] : -INFINITY;
+        float vs = fmaxf(x0, x1), vp = vs;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float us = __shfl_down(vs, o), up = __shfl_up(vp, o);  // own value when out of range
+            vs = fmaxf(vs, us);
+            vp = fmaxf(vp, up);
+        }
+        float ns = __shfl_down(vs, 1), np = __shfl_up(vp, 1);
+        if (l == 63) ns = -INFINITY;
+        if (l == 0) np = -INFINITY;
+        const float s1 = fmaxf(x1, ns), p0 = fmaxf(x0, np);
+        if (2 * l < nch) {
+            suf[2 * l] = fmaxf(x0, s1);
+            pre[2 * l] = p0;
+        }
+        if (2 * l + 1 < nch) {
+            suf[2 * l + 1] = s1;
+            pre[2 * l + 1] = fmaxf(x1, p0);
+        }
+    }
 }
 
 // One skewed row: stage it, then every wave of the workgroup takes chunks
@@ -181,7 +259,8 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
                                               const float* __restrict__ in4, const int* __restrict__ target,
                                               const float* __restrict__ weight, const uint32_t* __restrict__ fl,
                                               const float* __restrict__ Sbuf, const float4* __restrict__ rtab,
-                                              float povh, float* __restrict__ arena, float* row) {
+                                              float povh, float* __restrict__ arena, float* row,
+                                              unsigned long long* __restrict__ steps) {
     const SecDesc sd = secs[w.sec];
     const int r = w.row;
     const uint32_t v = fl[sd.rows + r];
@@ -190,11 +269,16 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
     if (L <= 0) return;  // uniform; the work list holds only rows with L > 0
     stage_row(row, sd, r, f, L, in4, target, weight, Sbuf, kPadBits);
     __syncthreads();
+    float* suf = row + L + kPadBits;  // past the right guard band
+    float* pre = suf + kChunksMax;
+    row_chunk_max<kWaves>(row, L, suf, pre);
+    __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nchunk = (L + 63) >> 6;
     float* so = arena + sd.so + (size_t)r * sd.dim_i + f;
+    int nblk = 0;  // 32-step blocks this wave ran (both directions)
     for (int c = wave; c < nchunk; c += kWaves) {
         const int j0 = c << 6;
         const int jj = j0 + lane;
@@ -204,12 +288,14 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
         // right scan: k = j+1 .. last-1
         const int dmaxR = L - 1 - j0;
         const int cvR = scan_dir<+1>(row + jj, h, rtab, valid ? L - 1 - jj : 0, full ? L - 1 - (j0 + 63) : 0,
-                                     (dmaxR + 31) & ~31);
+                                     (dmaxR + 31) & ~31, suf, jj, L, nblk);
         // left scan: k = j-1 .. first
         const int dmaxL = full ? j0 + 63 : L - 1;
-        const int cvL = scan_dir<-1>(row + jj, h, rtab, valid ? jj : 0, full ? j0 : 0, (dmaxL + 31) & ~31);
+        const int cvL = scan_dir<-1>(row + jj, h, rtab, valid ? jj : 0, full ? j0 : 0, (dmaxL + 31) & ~31, pre,
+                                     jj, L, nblk);
         if (valid) so[jj] = (float)(cvR + cvL) * sd.R;
     }
+    if (steps && lane == 0 && nblk) atomicAdd(steps, 2048ull * (unsigned long long)nblk);  // 32 steps x 64 lanes
     __syncthreads();
 }
 
@@ -223,10 +309,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __r
                                                               const uint32_t* __restrict__ fl,
                                                               const float* __restrict__ Sbuf,
                                                               const float4* __restrict__ rtab, float povh,
-                                                              float* __restrict__ arena) {
+                                                              float* __restrict__ arena,
+                                                              unsigned long long* __restrict__ steps) {
     extern __shared__ float lds[];
     sdem_bits_row<kWaves>(work[blockIdx.x], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
-                          lds + kPadBits);
+                          lds + kPadBits, steps);
 }
 
 // Persistent: a fixed grid pulls rows off the (cost-sorted) work list with
@@ -239,16 +326,17 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kOc
     const SecDesc* __restrict__ secs, const RowWork* __restrict__ work, int nwork, int* __restrict__ counter,
     const float* __restrict__ in4, const int* __restrict__ target, const float* __restrict__ weight,
     const uint32_t* __restrict__ fl, const float* __restrict__ Sbuf, const float4* __restrict__ rtab, float povh,
-    float* __restrict__ arena) {
+    float* __restrict__ arena, unsigned long long* __restrict__ steps) {
     extern __shared__ float lds[];
     __shared__ int item;
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(counter, 1);
         __syncthreads();
-        const int it = item;
+        const int it = __builtin_amdgcn_readfirstlane(item);  // the same value in every lane
         __syncthreads();  // every lane has read `item` before thread 0 overwrites it
         if (it >= nwork) break;
-        sdem_bits_row<kWaves>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena, lds + kPadBits);
+        sdem_bits_row<kWaves>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena, lds + kPadBits,
+                              steps);
     }
 }
 
@@ -258,10 +346,11 @@ __global__ void k_zero_int(int* p, int n) {
 }
 
 hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
-                                const float4* rtab, float povh, const SdemTuning& t, int* counter, hipStream_t s) {
+                                const float4* rtab, float povh, const SdemTuning& t, int* counter,
+                                unsigned long long* steps, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
     if (p.maxL > 4096) return launch_op_sdem(p, in4, Sbuf, arena, rtab, povh, 2, s);
-    const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPadBits);
+    const size_t lds = sdem_bits_lds(p.maxL);
     if (t.persistent_wgs_per_cu > 0 && counter) {
         int dev = 0, ncu = 0;
         hipError_t e = hipGetDevice(&dev);
@@ -272,7 +361,7 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
         const int grid = std::min(p.nwork, ncu * t.persistent_wgs_per_cu);
 #define SKE_PERS(W, O)                                                                                           \
     hipLaunchKernelGGL((k_op_sdem_bits_persistent<W, O>), dim3(grid), dim3(64 * W), lds, s, p.secs, p.work,       \
-                       p.nwork, counter, in4, p.target, p.weight, p.fl, Sbuf, rtab, povh, arena)
+                       p.nwork, counter, in4, p.target, p.weight, p.fl, Sbuf, rtab, povh, arena, steps)
         if (t.occupancy >= 8) {
             if (t.waves == 2) SKE_PERS(2, 8); else if (t.waves == 8) SKE_PERS(8, 8); else SKE_PERS(4, 8);
         } else {
@@ -283,7 +372,7 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
     }
 #define SKE_GRID(W)                                                                                             \
     hipLaunchKernelGGL(k_op_sdem_bits<W>, dim3(p.nwork), dim3(64 * W), lds, s, p.secs, p.work, in4, p.target, \
-                       p.weight, p.fl, Sbuf, rtab, povh, arena)
+                       p.weight, p.fl, Sbuf, rtab, povh, arena, steps)
     if (t.waves == 2) SKE_GRID(2); else if (t.waves == 8) SKE_GRID(8); else SKE_GRID(4);
 #undef SKE_GRID
     return hipGetLastError();
@@ -293,9 +382,9 @@ hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf,
                           const float4* rtab, float povh, int variant, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
     if (variant == 0 && p.maxL <= 4096) {
-        const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPadBits);
+        const size_t lds = sdem_bits_lds(p.maxL);
         hipLaunchKernelGGL(k_op_sdem_bits<4>, dim3(p.nwork), dim3(256), lds, s, p.secs, p.work, in4, p.target,
-                           p.weight, p.fl, Sbuf, rtab, povh, arena);
+                           p.weight, p.fl, Sbuf, rtab, povh, arena, nullptr);
         return hipGetLastError();
     }
     const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPad);

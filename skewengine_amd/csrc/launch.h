@@ -49,9 +49,11 @@ hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf,
                           const float4* rtab, float povh, int variant, hipStream_t s);
 // Bit-block viewshed with an explicit launch shape; `counter` (one int of
 // device memory) feeds the persistent grid.
+// `steps`, when not null, receives += the lane-steps the scans executed
+// (32-step blocks x 64 lanes; the early exit skips the rest).
 hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
                                 const float4* rtab, float povh, const SdemTuning& t, int* counter,
-                                hipStream_t s);
+                                unsigned long long* steps, hipStream_t s);
 // tab: cos/sin(2 pi t/64) interleaved (128 doubles) then cos(2 pi t/32) (32).
 hipError_t launch_op_blur(const DevPlan& p, const float* in4, const float* Sbuf, float* arena,
                           const double* tab, hipStream_t s);

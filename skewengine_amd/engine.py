@@ -126,6 +126,13 @@ class SkewEngine:
     def profile_reset(self):
         check(lib().ske_profile_reset(self._h))
 
+    def counter(self, name):
+        """Work counter accumulated since the last profile reset (e.g.
+        "op_sdem_lane_steps" with OPT_COUNT_STEPS on)."""
+        v = ctypes.c_double(0)
+        check(lib().ske_counter_get(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
 
 class SkewBatch:
     """Many square grids of uneven sizes through one all-sector run.
@@ -193,6 +200,13 @@ class SkewBatch:
 
     def profile_reset(self):
         check(lib().ske_batch_profile_reset(self._h))
+
+    def counter(self, name):
+        """Work counter accumulated since the last profile reset (e.g.
+        "op_sdem_lane_steps" with OPT_COUNT_STEPS on)."""
+        v = ctypes.c_double(0)
+        check(lib().ske_batch_counter_get(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def autotune(self, kernel_id=KERNEL_SDEM, povh=POVH_DEFAULT):
         """Pick the viewshed launch shape: returns (waves, persistent_wgs_per_cu, occupancy, best_ms)."""
