@@ -28,10 +28,11 @@ sys.path.insert(0, ROOT)
 # of v_sub/v_max, consistent with that figure.
 PEAK_VALU_TOPS = 157.3 / 2
 OPS_PER_STEP = 4  # sub, divide, compare, max per ray step (src/kernelSDEM.h:191-196)
-# VALU instructions per executed lane-step of the bit-block scan loop in the
-# gfx950 build: 246 per 32-step block (6 per step, then the popcount flush,
-# the early-exit bound and the loop control).
-VALU_PER_LANE_STEP = 246 / 32
+# VALU instructions per executed lane-step of the compacting bit-block scan
+# loop in the gfx950 build: about 270 per 32-step block of one slot (6 per
+# step, then the popcount flush, the end-of-row mask, the early-exit bound
+# and the retire/compaction bookkeeping).
+VALU_PER_LANE_STEP = 270 / 32
 HBM_PEAK_GBS = 8000.0
 
 
@@ -102,7 +103,9 @@ def bench_viewshed_batch(args, ske, synth):
         steps_total += st
         pts_sectors += n * n * 180.0
     waves, pers, occ, _ = batch.autotune(ske.KERNEL_SDEM, ske.POVH_DEFAULT)
-    _progress("autotuned: %d waves/workgroup, %d persistent workgroups/CU, occupancy hint %d" % (waves, pers, occ))
+    group = batch.get_option(_lib.OPT_SDEM_GROUP)
+    _progress("autotuned: %d waves/workgroup, %d persistent workgroups/CU, occupancy hint %d, chunk group %d"
+              % (waves, pers, occ, group))
     batch.set_option(_lib.OPT_PROFILE, 1)  # HIP events around every launch on the batch stream
 
     for _ in range(args.warmup):
@@ -121,7 +124,6 @@ def bench_viewshed_batch(args, ske, synth):
     batch.outputs()  # device -> host copy of the 32 result maps
     t_out = time.perf_counter() - t_out
     k_ms, k_cnt = batch.profile("op_sdem")
-    achieved = OPS_PER_STEP * steps_total * args.steps / (k_ms / 1e3) / 1e12
     # One more (untimed) step with the kernels counting the ray steps their
     # lanes execute: the scans stop once no later target can be visible.
     batch.set_option(_lib.OPT_COUNT_STEPS, 1)
@@ -131,6 +133,10 @@ def bench_viewshed_batch(args, ske, synth):
     batch.set_option(_lib.OPT_COUNT_STEPS, 0)
     _progress("counted %.4e executed lane-steps (%.3f of the reference's ray steps)"
               % (lane_steps, lane_steps / steps_total))
+    # Roofline of the work the kernel performs: the scans stop once no later
+    # target can be visible, so the steps executed (whole 32-step blocks of 64
+    # lanes) replace the reference's sum L(L-1) as the algorithmic count.
+    achieved = OPS_PER_STEP * lane_steps * args.steps / (k_ms / 1e3) / 1e12
     result = {
         "metric": "Mpts*sectors/s (batched total viewshed)",
         "value": pts_sectors / 1e6 / sec_per_step,
@@ -147,12 +153,14 @@ def bench_viewshed_batch(args, ske, synth):
                      "frac": achieved / PEAK_VALU_TOPS, "traffic": _traffic("op_sdem"),
                      "kernel": "op_sdem", "kernel_ms_total": k_ms, "launches": k_cnt,
                      "ray_steps_per_s": steps_total * args.steps / (k_ms / 1e3),
+                     "reference_step_equivalent_frac": OPS_PER_STEP * steps_total * args.steps / (k_ms / 1e3)
+                                                       / 1e12 / PEAK_VALU_TOPS,
                      "executed_lane_steps_per_step": lane_steps,
                      "executed_over_reference_steps": lane_steps / steps_total,
                      "issued_valu_frac": VALU_PER_LANE_STEP * lane_steps * args.steps / (k_ms / 1e3) / 1e12
                                          / PEAK_VALU_TOPS,
                      "autotuned": {"waves_per_workgroup": waves, "persistent_workgroups_per_cu": pers,
-                                   "waves_per_simd_hint": occ}},
+                                   "waves_per_simd_hint": occ, "chunk_group": group}},
     }
     batch.close()
     return result, steps_total, pts_sectors

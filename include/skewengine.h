@@ -138,7 +138,11 @@ int ske_synchronize(ske_engine *e);
 #define SKE_OPT_COUNT_STEPS 9     /* 1: the all-sector bit-block viewshed counts the ray steps its
                                    * lanes execute (counter "op_sdem_lane_steps"; the scans stop
                                    * early once no later target can be visible) */
+#define SKE_OPT_SDEM_GROUP 10      /* viewshed: 4 (default) groups of four 64-observer chunks per
+                                   * wave whose finished scans are replaced by the group's remaining
+                                   * ones (lane compaction); 1 one chunk per wave */
 int ske_set_option(ske_engine *e, int option, long long value);
+int ske_get_option(ske_engine *e, int option, long long *value);
 /* Accumulated HIP-event time of the launches named `name` since the last
  * reset (profiling option on): total milliseconds and launch count. */
 int ske_profile_get(ske_engine *e, const char *name, double *total_ms, int *count);
@@ -171,6 +175,7 @@ int ske_batch_output(ske_batch *b, int index, float *output);
 int ske_batch_output_device(ske_batch *b, int index, const float **d_output);
 int ske_batch_synchronize(ske_batch *b);
 int ske_batch_set_option(ske_batch *b, int option, long long value);
+int ske_batch_get_option(ske_batch *b, int option, long long *value);
 int ske_batch_profile_get(ske_batch *b, const char *name, double *total_ms, int *count);
 int ske_batch_counter_get(ske_batch *b, const char *name, double *value);
 int ske_batch_profile_reset(ske_batch *b);

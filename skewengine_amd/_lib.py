@@ -15,6 +15,7 @@ OPT_SDEM_VARIANT, OPT_ARENA_BYTES, OPT_PROFILE, OPT_GRAPH, OPT_SDEM_WAVES, OPT_S
 OPT_SDEM_OCCUPANCY = 7
 OPT_RADON_LOCAL = 8
 OPT_COUNT_STEPS = 9
+OPT_SDEM_GROUP = 10
 LEARNED_TAPS = 19105
 LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180
@@ -24,10 +25,11 @@ EXPORTS = [
     "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_create",
     "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
     "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_get_skew", "ske_set_skew_output",
-    "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option",
+    "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option", "ske_get_option",
     "ske_profile_get", "ske_counter_get", "ske_profile_reset", "ske_work",
     "ske_batch_create", "ske_batch_create_device", "ske_batch_destroy", "ske_batch_run", "ske_batch_output",
-    "ske_batch_output_device", "ske_batch_synchronize", "ske_batch_set_option", "ske_batch_profile_get",
+    "ske_batch_output_device", "ske_batch_synchronize", "ske_batch_set_option", "ske_batch_get_option",
+    "ske_batch_profile_get",
     "ske_batch_counter_get", "ske_batch_profile_reset", "ske_batch_autotune",
     "ske_fbp_create", "ske_fbp_destroy", "ske_fbp_upload", "ske_fbp_download", "ske_fbp_buffer", "ske_fbp_project",
     "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct", "ske_fbp_set_taps", "ske_fbp_get_taps",
@@ -95,6 +97,7 @@ def lib():
     L.ske_output_device.argtypes = [vp, P(vp)]
     L.ske_synchronize.argtypes = [vp]
     L.ske_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.ske_get_option.argtypes = [vp, ctypes.c_int, P(ctypes.c_longlong)]
     L.ske_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_counter_get.argtypes = [vp, ctypes.c_char_p, f64p]
     L.ske_profile_reset.argtypes = [vp]
@@ -107,6 +110,7 @@ def lib():
     L.ske_batch_output_device.argtypes = [vp, ctypes.c_int, P(vp)]
     L.ske_batch_synchronize.argtypes = [vp]
     L.ske_batch_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_longlong]
+    L.ske_batch_get_option.argtypes = [vp, ctypes.c_int, P(ctypes.c_longlong)]
     L.ske_batch_profile_get.argtypes = [vp, ctypes.c_char_p, f64p, i32p]
     L.ske_batch_counter_get.argtypes = [vp, ctypes.c_char_p, f64p]
     L.ske_batch_profile_reset.argtypes = [vp]

@@ -454,8 +454,9 @@ struct Core {
         char key[160];
         unsigned pbits = 0;
         memcpy(&pbits, &povh, sizeof pbits);
-        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%lld:%d", kernel_id, s0, s1, pbits, sdem_variant,
-                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, arena_bytes, radon_local ? 1 : 0);
+        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%d:%lld:%d:%d", kernel_id, s0, s1, pbits, sdem_variant,
+                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, tune.group, arena_bytes,
+                 radon_local ? 1 : 0, count_steps ? 1 : 0);
         if (!gexec || gkey != key) {
             drop_graph();
             SKE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
@@ -506,11 +507,16 @@ struct Core {
             tune.persistent_wgs_per_cu = (int)value;
             return SKE_OK;
         case SKE_OPT_SDEM_OCCUPANCY:
-            if (value != 0 && value != 8) return fail(SKE_EINVAL, "occupancy hint must be 0 or 8");
+            if (value != 0 && value != 6 && value != 8) return fail(SKE_EINVAL, "occupancy hint must be 0, 6 or 8");
             tune.occupancy = (int)value;
             return SKE_OK;
         case SKE_OPT_RADON_LOCAL:
             radon_local = value != 0;
+            drop_graph();
+            return SKE_OK;
+        case SKE_OPT_SDEM_GROUP:
+            if (value != 1 && value != 4) return fail(SKE_EINVAL, "viewshed chunk group must be 1 or 4");
+            tune.group = (int)value;
             drop_graph();
             return SKE_OK;
         case SKE_OPT_COUNT_STEPS:
@@ -521,6 +527,24 @@ struct Core {
         default:
             return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
         }
+    }
+
+    int get_option(int option, long long* value) {
+        if (!value) return fail(SKE_EINVAL, "value is NULL");
+        switch (option) {
+        case SKE_OPT_SDEM_VARIANT: *value = sdem_variant; break;
+        case SKE_OPT_ARENA_BYTES: *value = arena_bytes; break;
+        case SKE_OPT_PROFILE: *value = profile ? 1 : 0; break;
+        case SKE_OPT_GRAPH: *value = use_graph ? 1 : 0; break;
+        case SKE_OPT_SDEM_WAVES: *value = tune.waves; break;
+        case SKE_OPT_SDEM_PERSISTENT: *value = tune.persistent_wgs_per_cu; break;
+        case SKE_OPT_SDEM_OCCUPANCY: *value = tune.occupancy; break;
+        case SKE_OPT_RADON_LOCAL: *value = radon_local ? 1 : 0; break;
+        case SKE_OPT_COUNT_STEPS: *value = count_steps ? 1 : 0; break;
+        case SKE_OPT_SDEM_GROUP: *value = tune.group; break;
+        default: return fail(SKE_EINVAL, "unknown option " + std::to_string(option));
+        }
+        return SKE_OK;
     }
 
     int counter_get(const char* name, double* value) {
@@ -555,8 +579,8 @@ struct Core {
         DevBuf<float> scratch;
         SKE_TRY(scratch.ensure((size_t)sample->ip.so_total));
         const DevPlan v = sample->view();
-        const SdemTuning cands[] = {{4, 0, 0}, {8, 0, 0}, {4, 4, 0}, {8, 4, 0}, {2, 8, 0}, {4, 8, 8},
-                                    {8, 4, 8}, {2, 16, 8}, {4, 16, 8}, {8, 8, 8}};
+        const SdemTuning cands[] = {{4, 0, 0, 1}, {4, 8, 8, 1}, {4, 16, 8, 1}, {8, 8, 8, 1}, {2, 16, 8, 1},
+                                    {4, 0, 0, 4}, {8, 0, 0, 4}, {2, 0, 0, 4}, {4, 8, 6, 4}, {4, 8, 0, 4}};
         hipEvent_t a = take_event(), b = take_event();
         double best = 1e300;
         SdemTuning pick = tune;
@@ -895,6 +919,11 @@ int ske_set_option(ske_engine* e, int option, long long value) {
     return e->core.set_option(option, value);
 }
 
+int ske_get_option(ske_engine* e, int option, long long* value) {
+    if (!e) return fail(SKE_EINVAL, "engine is NULL");
+    return e->core.get_option(option, value);
+}
+
 int ske_profile_get(ske_engine* e, const char* name, double* total_ms, int* count) {
     if (!e || !name) return fail(SKE_EINVAL, "engine or name is NULL");
     return e->core.profile_get(name, total_ms, count);
@@ -1001,6 +1030,11 @@ int ske_batch_synchronize(ske_batch* b) {
 int ske_batch_set_option(ske_batch* b, int option, long long value) {
     if (!b) return fail(SKE_EINVAL, "batch is NULL");
     return b->core.set_option(option, value);
+}
+
+int ske_batch_get_option(ske_batch* b, int option, long long* value) {
+    if (!b) return fail(SKE_EINVAL, "batch is NULL");
+    return b->core.get_option(option, value);
 }
 
 int ske_batch_profile_get(ske_batch* b, const char* name, double* total_ms, int* count) {

@@ -129,9 +129,20 @@ __global__ __launch_bounds__(256) void k_op_sdem(const SecDesc* __restrict__ sec
 // ---------------------------------------------------------------------------
 constexpr int kPadBits = 128;
 constexpr int kChunksMax = 128;  // 32-cell chunks of a row of <= 4096 cells
+// Chunk-maximum tables of a staged row, each indexed by chunk c in
+// [-kTabLo, kChunksMax + kTabHi): level 0 cm[c] = max row[32c .. 32c+31],
+// level 1..3 the maxima of 2, 4 and 8 consecutive chunks starting at c,
+// then suffix and prefix maxima.  Chunks outside the row hold -inf (the
+// prefix table repeats the row maximum past the end).
+constexpr int kTabLo = 16, kTabHi = 32;
+constexpr int kTabN = kTabLo + kChunksMax + kTabHi;
+enum { kTabL0 = 0, kTabL1 = 1, kTabL2 = 2, kTabL3 = 3, kTabSuf = 4, kTabPre = 5, kTabs = 6 };
 
-// LDS of the bit-block kernels: guard band, row, guard band, chunk maxima.
-inline size_t sdem_bits_lds(int maxL) { return sizeof(float) * ((size_t)maxL + 2 * kPadBits + 2 * kChunksMax); }
+// LDS of the bit-block kernels: guard band, row, guard band, chunk tables,
+// and for lane compaction (group > 1) a list of 64 x group ints per wave.
+inline size_t sdem_bits_lds(int maxL, int waves = 4, int group = 1) {
+    return sizeof(float) * ((size_t)maxL + 2 * kPadBits + kTabs * kTabN + (group > 1 ? 64 * waves * group : 0));
+}
 
 __device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) {
     const int pc = __popc(w);
@@ -167,69 +178,101 @@ __device__ __forceinline__ uint32_t scan_block(const float* rowj, float h, const
     return w;
 }
 
-// Early exit.  After the steps up to db, no later target k of this lane can
-// be visible once an upper bound of its quotient is <= M: with a = Smax - h
-// (Smax the largest height among the remaining targets, taken from the
-// per-32-cell maxima `cmax`), every later a_k <= a by monotone rounding, so
-// q_k <= RN(a / (db+1)) for a >= 0 and q_k <= RN(a / myLen) for a < 0.  The
-// bound is evaluated with one-sided slack (2^-20 relative), which keeps it
-// above those rounded quotients.  When every lane of the wave is past its row
-// end or bounded, the scan stops: a lane with steps left has only invisible
-// steps ahead, so its trailing visible run closes and counts (cv = A); a
-// lane whose row ended keeps the usual correction of the padded steps.
+// Early exit.  After the steps up to db, no later target of a lane can be
+// visible once an upper bound of its quotient is <= M.  The remaining
+// targets (distances e = db+1 .. myLen) are split into four segments that
+// start at distances e, e+32, e+96 and e+224 (the last runs to the row end);
+// with a_i = (largest height of segment i) - h, every later a_k is <= the
+// a_i of its segment by monotone rounding, so every later quotient is
+//     <= max( max_i a_i / e_i ,  (max_i a_i) / L ).
+// (The second term covers a_i < 0, where a farther target has the larger
+// quotient; L exceeds every distance.  When some a_i >= 0 it is dominated by
+// the first term.)  Segment maxima come from the chunk tables over a
+// superset of each segment's cells: extra cells only raise the bound.  The
+// products use RN(1/e) and are rounded once, so the bound is raised by
+// 2^-20 of its magnitude to stay above the rounded quotients.  When every
+// lane of the wave is past its row end or bounded by its running maximum
+// M, the scan stops: a lane with steps left has only invisible steps ahead,
+// so its trailing visible run closes and counts (cv = A); a lane whose row
+// ended keeps the usual correction of the padded steps.
+// kClamp: clamp the nearest-target position into the row (callers whose
+// lanes may lie far past their row end); without it p must lie in
+// [-32, L + 31], which the chunk tables' padding covers.
+template <int kDir, bool kClamp = true>
+__device__ __forceinline__ float exit_bound(const float* __restrict__ tab, int jj, int L, float h, int db,
+                                            const float2* __restrict__ ytab, float invL) {
+    const int e = db + 1;
+    int p = jj + kDir * e;  // nearest remaining target
+    if (kClamp) p = p < 0 ? 0 : (p > L - 1 ? L - 1 : p);
+    const float* t = tab + kTabLo + (p >> 5);
+    float s1, s2, s3, s4;
+    if (kDir > 0) {  // cells [p, p+32), [p+32, p+96), [p+96, p+224), [p+224, end)
+        s1 = t[kTabL1 * kTabN];
+        s2 = t[kTabL2 * kTabN + 1];
+        s3 = t[kTabL3 * kTabN + 3];
+        s4 = t[kTabSuf * kTabN + 7];
+    } else {         // cells (p-32, p], (p-96, p-32], (p-224, p-96], [0, p-224]
+        s1 = t[kTabL1 * kTabN - 1];
+        s2 = t[kTabL2 * kTabN - 4];
+        s3 = t[kTabL3 * kTabN - 10];
+        s4 = t[kTabPre * kTabN - 7];
+    }
+    const float a1 = s1 - h, a2 = s2 - h, a3 = s3 - h, a4 = s4 - h;
+    const float amax = fmaxf(fmaxf(a1, a2), fmaxf(a3, a4));
+    float b = fmaxf(fmaxf(a1 * ytab[e].x, a2 * ytab[e + 32].x), fmaxf(a3 * ytab[e + 96].x, a4 * ytab[e + 224].x));
+    b = fmaxf(b, amax * invL);
+    return __builtin_fmaf(fabsf(b), 9.5367431640625e-7f, b);  // b + |b| 2^-20
+}
+
 template <int kDir>
-__device__ __forceinline__ bool scan_done(const float* __restrict__ cmax, int jj, int L, float h, float rl,
+__device__ __forceinline__ bool scan_done(const float* __restrict__ tab, int jj, int L, float h, float invL,
                                           const float2* __restrict__ ytab, int myLen, int db, float M) {
-    int pos = jj + kDir * (db + 1);
-    pos = pos < 0 ? 0 : (pos > L - 1 ? L - 1 : pos);  // lanes with myLen <= db are done anyway
-    const float a = cmax[pos >> 5] - h;
-    const float bpos = (a * ytab[db + 1].x) * 1.00000095367431640625f;
-    const float bneg = (a * rl) * 0.99999904632568359375f;
-    const bool done = (myLen <= db) | ((a >= 0.0f ? bpos : bneg) <= M);
+    const bool done = (myLen <= db) | (exit_bound<kDir>(tab, jj, L, h, db, ytab, invL) <= M);
     // every lane of the wave is active here; a ballot keeps the exit uniform
     return __builtin_amdgcn_readfirstlane((int)(__builtin_amdgcn_ballot_w64(!done) != 0)) == 0;
 }
 
 template <int kDir>
 __device__ __forceinline__ int scan_dir(const float* rowj, float h, const float4* __restrict__ rtab, int myLen,
-                                        int dfull, int D, const float* __restrict__ cmax, int jj, int L,
+                                        int dfull, int D, const float* __restrict__ tab, int jj, int L,
                                         int& nblk) {
     const float2* __restrict__ ytab = reinterpret_cast<const float2*>(rtab + kMaxDim + 1);
-    const float rl = 1.0f / (float)(myLen > 0 ? myLen : 1);
+    const float invL = ytab[L].x;
     float M = -2000.0f;
     int A = 1, s = 0;
     int d0 = 0;
     int e1 = dfull, e2 = D;  // loop ends, lowered to the current step when the scan stops
     for (; d0 + 32 <= e1; d0 += 32) {
         flush_block(scan_block<kDir, false>(rowj, h, rtab, d0, myLen, M), d0, A, s);
-        if (scan_done<kDir>(cmax, jj, L, h, rl, ytab, myLen, d0 + 32, M)) e1 = e2 = d0 + 32;
+        if (scan_done<kDir>(tab, jj, L, h, invL, ytab, myLen, d0 + 32, M)) e1 = e2 = d0 + 32;
     }
     for (; d0 < e2; d0 += 32) {
         flush_block(scan_block<kDir, true>(rowj, h, rtab, d0, myLen, M), d0, A, s);
-        if (scan_done<kDir>(cmax, jj, L, h, rl, ytab, myLen, d0 + 32, M)) e2 = d0 + 32;
+        if (scan_done<kDir>(tab, jj, L, h, invL, ytab, myLen, d0 + 32, M)) e2 = d0 + 32;
     }
     nblk += d0 >> 5;
     return myLen > d0 ? A : A - ((d0 + 1) * (d0 + 1) - s * s);
 }
 
-// Per-32-cell maxima of the staged row (cm[c] = max row[32c .. 32c+31]),
-// then their suffix maxima (suf) and prefix maxima (pre), for the early exit
-// of the right and left scans.  Rows of at most 4096 cells: <= 128 chunks.
+// Build the chunk tables of a staged row of L <= 4096 cells (see kTabN).
+// Ends with a barrier.
 template <int kWaves>
-__device__ __forceinline__ void row_chunk_max(const float* row, int L, float* suf, float* pre) {
+__device__ __forceinline__ void row_tables(const float* row, int L, float* tab) {
     const int nch = (L + 31) >> 5;
-    for (int t0 = 0; t0 < nch * 32; t0 += 64 * kWaves) {
-        const int t = t0 + (int)threadIdx.x;
+    float* t0 = tab + kTabLo;
+    for (int i = threadIdx.x; i < kTabs * kTabN; i += 64 * kWaves) tab[i] = -INFINITY;
+    __syncthreads();
+    for (int t0i = 0; t0i < nch * 32; t0i += 64 * kWaves) {
+        const int t = t0i + (int)threadIdx.x;
         float v = t < L ? row[t] : -INFINITY;
 #pragma unroll
         for (int o = 16; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-        if ((t & 31) == 0 && t < nch * 32) suf[t >> 5] = v;
+        if ((t & 31) == 0 && t < nch * 32) t0[kTabL0 * kTabN + (t >> 5)] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {  // one wave: two chunks per lane
+    if (threadIdx.x < 64) {  // one wave: suffix and prefix maxima, two chunks per lane
         const int l = threadIdx.x;
-        const float x0 = 2 * l < nch ? suf[2 * l] : -INFINITY;
-        const float x1 = 2 * l + 1 < nch ? suf[2 * l + 1] : -INFINITY;
+        const float x0 = t0[kTabL0 * kTabN + 2 * l], x1 = t0[kTabL0 * kTabN + 2 * l + 1];  // -inf past nch
         float vs = fmaxf(x0, x1), vp = vs;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -240,27 +283,174 @@ __device__ __forceinline__ void row_chunk_max(const float* row, int L, float* su
         float ns = __shfl_down(vs, 1), np = __shfl_up(vp, 1);
         if (l == 63) ns = -INFINITY;
         if (l == 0) np = -INFINITY;
-        const float s1 = fmaxf(x1, ns), p0 = fmaxf(x0, np);
-        if (2 * l < nch) {
-            suf[2 * l] = fmaxf(x0, s1);
-            pre[2 * l] = p0;
+        const float s1 = fmaxf(x1, ns), p0 = fmaxf(x0, np), p1 = fmaxf(x1, p0);
+        t0[kTabSuf * kTabN + 2 * l] = fmaxf(x0, s1);
+        t0[kTabSuf * kTabN + 2 * l + 1] = s1;
+        t0[kTabPre * kTabN + 2 * l] = p0;
+        t0[kTabPre * kTabN + 2 * l + 1] = p1;
+        if (l < kTabHi) t0[kTabPre * kTabN + kChunksMax + l] = __shfl(p1, 63);  // whole-row maximum
+    }
+    for (int lev = 1; lev <= 3; ++lev) {  // level lev from level lev-1: max of two runs 2^(lev-1) apart
+        const int off = 1 << (lev - 1);
+        for (int i = threadIdx.x; i < kTabN - off; i += 64 * kWaves)
+            tab[lev * kTabN + i] = fmaxf(tab[(lev - 1) * kTabN + i], tab[(lev - 1) * kTabN + i + off]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lane compaction.  With the early exit a wave still runs until its longest
+// scan stops, while most of its lanes are done.  A wave therefore takes a
+// group of kG chunks (kG x 64 scans of one direction, all starting at d = 1)
+// held in kG register "slots" and steps all of them with the same uniform d.
+// After each 32-step block, lanes whose scan is over retire (write their cv)
+// and, once the live scans fit in fewer slots, they are packed into the
+// lowest slots in (slot, lane) order: the p-th live scan moves to slot p/64,
+// lane p%64.  Per-lane state moves with ds_bpermute; the source of each
+// destination comes from a small per-wave LDS index list.  The p-th live
+// scan of a slot k satisfies p < 64(k+1), so destination slot k2 draws only
+// from slots >= k2 and the packing can run in place, lowest slot first.
+// A lane's steps past its row end are recorded as visible by or-ing the low
+// bits of its block word; retired and empty lanes read from a position whose
+// remaining steps stay inside the staged row and guard bands.  The
+// right-scan pass stores cv as an integer in the output cell; the left-scan
+// pass of the same group (same wave, later) adds its cv and stores the float
+// (float)(cvR + cvL) * R, the same value as the one-chunk path.
+// ---------------------------------------------------------------------------
+template <int kDir, int kG>
+__device__ __forceinline__ void scan_group(const float* row, int L, float povh, int c0, int nck,
+                                           const float* __restrict__ tab, const float4* __restrict__ rtab,
+                                           float* __restrict__ so, float R, int* __restrict__ idx, int& nblk) {
+    const float2* __restrict__ ytab = reinterpret_cast<const float2*>(rtab + kMaxDim + 1);
+    int* __restrict__ soi = reinterpret_cast<int*>(so);
+    const float invL = ytab[L].x;
+    const int lane = threadIdx.x & 63;
+    const int safe = kDir > 0 ? 0 : L - 1;  // reads row[safe + kDir*d], d <= L + 31: inside the guard bands
+    int jj[kG], A[kG], s[kG];
+    float h[kG], M[kG];
+    uint64_t act[kG];  // live scans of each slot (wave-uniform masks)
+#pragma unroll
+    for (int k = 0; k < kG; ++k) {
+        const int j = 64 * (c0 + k) + lane;
+        const int myLen = kDir > 0 ? L - 1 - j : j;
+        const bool in = k < nck && j < L;
+        if (in && myLen <= 0) {  // no targets in this direction: cv = 0
+            if (kDir > 0)
+                soi[j] = 0;
+            else
+                so[j] = (float)soi[j] * R;
         }
-        if (2 * l + 1 < nch) {
-            suf[2 * l + 1] = s1;
-            pre[2 * l + 1] = fmaxf(x1, p0);
+        const bool a = in && myLen > 0;
+        act[k] = __builtin_amdgcn_ballot_w64(a);
+        jj[k] = a ? j : safe;
+        h[k] = row[jj[k]] + povh;
+        M[k] = -2000.0f;
+        A[k] = 1;
+        s[k] = 0;
+    }
+    int ns = nck;
+    for (int d0 = 0; ns > 0; d0 += 32) {
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+            if (k < ns) {
+                uint32_t w = scan_block<kDir, false>(row + jj[k], h[k], rtab, d0, 0, M[k]);
+                const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
+                // steps past the row end (the last e of the block) are recorded as visible
+                const int e = min(max(d0 + 32 - myLen, 0), 31);
+                w |= (1u << e) - 1u;
+                flush_block(w, d0, A[k], s[k]);
+            }
+        }
+        const int db = d0 + 32;
+        nblk += ns;
+        int T = 0;
+#pragma unroll
+        for (int k = 0; k < kG; ++k) {
+            if (k < ns) {
+                const bool a = (act[k] >> lane) & 1;
+                const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
+                // active lanes and empty lanes (jj = safe) keep the bound's table reads in range
+                const bool done =
+                    (myLen <= db) | (exit_bound<kDir, false>(tab, jj[k], L, h[k], db, ytab, invL) <= M[k]);
+                if (a && done) {
+                    const int cv = myLen > db ? A[k] : A[k] - ((db + 1) * (db + 1) - s[k] * s[k]);
+                    if (kDir > 0)
+                        soi[jj[k]] = cv;
+                    else
+                        so[jj[k]] = (float)(cv + soi[jj[k]]) * R;
+                }
+                const bool live = a && !done;
+                act[k] = __builtin_amdgcn_ballot_w64(live);
+                jj[k] = live ? jj[k] : safe;
+                T += (int)__popcll(act[k]);
+            }
+        }
+        T = __builtin_amdgcn_readfirstlane(T);
+        const int nn = (T + 63) >> 6;
+        if (nn < ns) {
+            int base = 0;
+#pragma unroll
+            for (int k = 0; k < kG; ++k) {
+                if (k < ns) {
+                    const uint64_t m = act[k];
+                    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    if ((m >> lane) & 1) idx[base + below] = 64 * k + lane;
+                    base += (int)__popcll(m);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k2 = 0; k2 < kG; ++k2) {
+                if (k2 < nn) {
+                    const int p = 64 * k2 + lane;
+                    const int src = p < T ? idx[p] : -1;
+                    const int sk = src >> 6;          // -1 for lanes left empty
+                    const int sa = (src & 63) << 2;   // ds_bpermute byte address
+                    int nj = safe, nA = 1, nS = 0;
+                    float nh = 0.0f, nM = -2000.0f;
+#pragma unroll
+                    for (int k = k2; k < kG; ++k) {
+                        if (k < ns) {
+                            const int tj = __builtin_amdgcn_ds_bpermute(sa, jj[k]);
+                            const int tA = __builtin_amdgcn_ds_bpermute(sa, A[k]);
+                            const int tS = __builtin_amdgcn_ds_bpermute(sa, s[k]);
+                            const int th = __builtin_amdgcn_ds_bpermute(sa, __float_as_int(h[k]));
+                            const int tM = __builtin_amdgcn_ds_bpermute(sa, __float_as_int(M[k]));
+                            if (sk == k) {
+                                nj = tj;
+                                nA = tA;
+                                nS = tS;
+                                nh = __int_as_float(th);
+                                nM = __int_as_float(tM);
+                            }
+                        }
+                    }
+                    jj[k2] = nj;
+                    A[k2] = nA;
+                    s[k2] = nS;
+                    h[k2] = nh;
+                    M[k2] = nM;
+                    act[k2] = __builtin_amdgcn_ballot_w64(p < T);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            ns = nn;
         }
     }
 }
 
 // One skewed row: stage it, then every wave of the workgroup takes chunks
-// of 64 observers.  Ends with a barrier so the caller may restage LDS.
-template <int kWaves>
+// of 64 observers (kG = 1), or groups of kG chunks with lane compaction,
+// handed out by an LDS counter.  Ends with a barrier so the caller may
+// restage LDS.
+template <int kWaves, int kG>
 __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __restrict__ secs,
                                               const float* __restrict__ in4, const int* __restrict__ target,
                                               const float* __restrict__ weight, const uint32_t* __restrict__ fl,
                                               const float* __restrict__ Sbuf, const float4* __restrict__ rtab,
                                               float povh, float* __restrict__ arena, float* row,
-                                              unsigned long long* __restrict__ steps) {
+                                              unsigned long long* __restrict__ steps, int* gnext) {
     const SecDesc sd = secs[w.sec];
     const int r = w.row;
     const uint32_t v = fl[sd.rows + r];
@@ -268,18 +458,30 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
     const int L = fl_last(v) - f;
     if (L <= 0) return;  // uniform; the work list holds only rows with L > 0
     stage_row(row, sd, r, f, L, in4, target, weight, Sbuf, kPadBits);
+    if (kG > 1 && threadIdx.x == 0) *gnext = kWaves;
     __syncthreads();
-    float* suf = row + L + kPadBits;  // past the right guard band
-    float* pre = suf + kChunksMax;
-    row_chunk_max<kWaves>(row, L, suf, pre);
-    __syncthreads();
+    float* tab = row + L + kPadBits;  // past the right guard band
+    row_tables<kWaves>(row, L, tab);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nchunk = (L + 63) >> 6;
     float* so = arena + sd.so + (size_t)r * sd.dim_i + f;
     int nblk = 0;  // 32-step blocks this wave ran (both directions)
-    for (int c = wave; c < nchunk; c += kWaves) {
+    if (kG > 1) {
+        int* idx = reinterpret_cast<int*>(tab + kTabs * kTabN) + wave * 64 * kG;
+        const int ngroups = (nchunk + kG - 1) / kG;
+        for (int g = wave; g < ngroups;) {
+            const int c0 = g * kG;
+            const int nck = min(kG, nchunk - c0);
+            scan_group<+1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, idx, nblk);
+            scan_group<-1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, idx, nblk);
+            int ng = 0;
+            if (lane == 0) ng = atomicAdd(gnext, 1);
+            g = __builtin_amdgcn_readfirstlane(ng);
+        }
+    }
+    for (int c = wave; kG == 1 && c < nchunk; c += kWaves) {
         const int j0 = c << 6;
         const int jj = j0 + lane;
         const bool valid = jj < L;
@@ -288,10 +490,10 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
         // right scan: k = j+1 .. last-1
         const int dmaxR = L - 1 - j0;
         const int cvR = scan_dir<+1>(row + jj, h, rtab, valid ? L - 1 - jj : 0, full ? L - 1 - (j0 + 63) : 0,
-                                     (dmaxR + 31) & ~31, suf, jj, L, nblk);
+                                     (dmaxR + 31) & ~31, tab, jj, L, nblk);
         // left scan: k = j-1 .. first
         const int dmaxL = full ? j0 + 63 : L - 1;
-        const int cvL = scan_dir<-1>(row + jj, h, rtab, valid ? jj : 0, full ? j0 : 0, (dmaxL + 31) & ~31, pre,
+        const int cvL = scan_dir<-1>(row + jj, h, rtab, valid ? jj : 0, full ? j0 : 0, (dmaxL + 31) & ~31, tab,
                                      jj, L, nblk);
         if (valid) so[jj] = (float)(cvR + cvL) * sd.R;
     }
@@ -300,7 +502,7 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
 }
 
 // One workgroup per row of the work list.
-template <int kWaves>
+template <int kWaves, int kG>
 __global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __restrict__ secs,
                                                               const RowWork* __restrict__ work,
                                                               const float* __restrict__ in4,
@@ -312,8 +514,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __r
                                                               float* __restrict__ arena,
                                                               unsigned long long* __restrict__ steps) {
     extern __shared__ float lds[];
-    sdem_bits_row<kWaves>(work[blockIdx.x], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
-                          lds + kPadBits, steps);
+    __shared__ int gnext;
+    sdem_bits_row<kWaves, kG>(work[blockIdx.x], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
+                              lds + kPadBits, steps, &gnext);
 }
 
 // Persistent: a fixed grid pulls rows off the (cost-sorted) work list with
@@ -321,22 +524,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __r
 // the counter passes nwork.  The counter is cleared by the launcher.
 // kOcc > 1 asks the compiler for at least kOcc waves per SIMD (fewer VGPRs,
 // less software pipelining of the LDS loads, more waves to hide latency).
-template <int kWaves, int kOcc>
+template <int kWaves, int kOcc, int kG>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kOcc, 8))) void k_op_sdem_bits_persistent(
     const SecDesc* __restrict__ secs, const RowWork* __restrict__ work, int nwork, int* __restrict__ counter,
     const float* __restrict__ in4, const int* __restrict__ target, const float* __restrict__ weight,
     const uint32_t* __restrict__ fl, const float* __restrict__ Sbuf, const float4* __restrict__ rtab, float povh,
     float* __restrict__ arena, unsigned long long* __restrict__ steps) {
     extern __shared__ float lds[];
-    __shared__ int item;
+    __shared__ int item, gnext;
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(counter, 1);
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(item);  // the same value in every lane
         __syncthreads();  // every lane has read `item` before thread 0 overwrites it
         if (it >= nwork) break;
-        sdem_bits_row<kWaves>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena, lds + kPadBits,
-                              steps);
+        sdem_bits_row<kWaves, kG>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
+                                  lds + kPadBits, steps, &gnext);
     }
 }
 
@@ -350,7 +553,8 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
                                 unsigned long long* steps, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
     if (p.maxL > 4096) return launch_op_sdem(p, in4, Sbuf, arena, rtab, povh, 2, s);
-    const size_t lds = sdem_bits_lds(p.maxL);
+    const int G = t.group > 1 ? 4 : 1;
+    const size_t lds = sdem_bits_lds(p.maxL, t.waves, G);
     if (t.persistent_wgs_per_cu > 0 && counter) {
         int dev = 0, ncu = 0;
         hipError_t e = hipGetDevice(&dev);
@@ -359,21 +563,33 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, counter, 1);
         const int grid = std::min(p.nwork, ncu * t.persistent_wgs_per_cu);
-#define SKE_PERS(W, O)                                                                                           \
-    hipLaunchKernelGGL((k_op_sdem_bits_persistent<W, O>), dim3(grid), dim3(64 * W), lds, s, p.secs, p.work,       \
+#define SKE_PERS(W, O, G)                                                                                        \
+    hipLaunchKernelGGL((k_op_sdem_bits_persistent<W, O, G>), dim3(grid), dim3(64 * W), lds, s, p.secs, p.work,    \
                        p.nwork, counter, in4, p.target, p.weight, p.fl, Sbuf, rtab, povh, arena, steps)
+#define SKE_PERS_W(O, G)                                                                                         \
+    if (t.waves == 2) SKE_PERS(2, O, G); else if (t.waves == 8) SKE_PERS(8, O, G); else SKE_PERS(4, O, G)
+#define SKE_PERS_G(O)                                                                                            \
+    if (G == 4) { SKE_PERS_W(O, 4); } else { SKE_PERS_W(O, 1); }
         if (t.occupancy >= 8) {
-            if (t.waves == 2) SKE_PERS(2, 8); else if (t.waves == 8) SKE_PERS(8, 8); else SKE_PERS(4, 8);
+            SKE_PERS_G(8);
+        } else if (t.occupancy >= 6) {
+            SKE_PERS_G(6);
         } else {
-            if (t.waves == 2) SKE_PERS(2, 1); else if (t.waves == 8) SKE_PERS(8, 1); else SKE_PERS(4, 1);
+            SKE_PERS_G(1);
         }
+#undef SKE_PERS_G
+#undef SKE_PERS_W
 #undef SKE_PERS
         return hipGetLastError();
     }
-#define SKE_GRID(W)                                                                                             \
-    hipLaunchKernelGGL(k_op_sdem_bits<W>, dim3(p.nwork), dim3(64 * W), lds, s, p.secs, p.work, in4, p.target, \
+#define SKE_GRID(W, G)                                                                                            \
+    hipLaunchKernelGGL((k_op_sdem_bits<W, G>), dim3(p.nwork), dim3(64 * W), lds, s, p.secs, p.work, in4, p.target, \
                        p.weight, p.fl, Sbuf, rtab, povh, arena, steps)
-    if (t.waves == 2) SKE_GRID(2); else if (t.waves == 8) SKE_GRID(8); else SKE_GRID(4);
+    if (G == 4) {
+        if (t.waves == 2) SKE_GRID(2, 4); else if (t.waves == 8) SKE_GRID(8, 4); else SKE_GRID(4, 4);
+    } else {
+        if (t.waves == 2) SKE_GRID(2, 1); else if (t.waves == 8) SKE_GRID(8, 1); else SKE_GRID(4, 1);
+    }
 #undef SKE_GRID
     return hipGetLastError();
 }
@@ -383,7 +599,7 @@ hipError_t launch_op_sdem(const DevPlan& p, const float* in4, const float* Sbuf,
     if (p.nwork == 0) return hipSuccess;
     if (variant == 0 && p.maxL <= 4096) {
         const size_t lds = sdem_bits_lds(p.maxL);
-        hipLaunchKernelGGL(k_op_sdem_bits<4>, dim3(p.nwork), dim3(256), lds, s, p.secs, p.work, in4, p.target,
+        hipLaunchKernelGGL((k_op_sdem_bits<4, 1>), dim3(p.nwork), dim3(256), lds, s, p.secs, p.work, in4, p.target,
                            p.weight, p.fl, Sbuf, rtab, povh, arena, nullptr);
         return hipGetLastError();
     }

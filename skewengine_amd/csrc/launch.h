@@ -26,6 +26,7 @@ struct SdemTuning {
     int waves = 4;                 // waves per workgroup (observer chunks of one row in parallel): 2, 4 or 8
     int persistent_wgs_per_cu = 0; // 0: one workgroup per row; >0: persistent grid of this many per CU
     int occupancy = 0;             // persistent grid: 8 = compile for 8 waves per SIMD (fewer VGPRs)
+    int group = 4;                 // 1: one 64-observer chunk per wave; 4: groups of 4 chunks, lane compaction
 };
 
 // Graph-safe clears (kernels, not hipMemsetAsync).

@@ -117,6 +117,11 @@ class SkewEngine:
     def set_option(self, option, value):
         check(lib().ske_set_option(self._h, int(option), int(value)))
 
+    def get_option(self, option):
+        v = ctypes.c_longlong(0)
+        check(lib().ske_get_option(self._h, int(option), ctypes.byref(v)))
+        return v.value
+
     def profile(self, name):
         ms = ctypes.c_double(0)
         cnt = ctypes.c_int(0)
@@ -191,6 +196,11 @@ class SkewBatch:
 
     def set_option(self, option, value):
         check(lib().ske_batch_set_option(self._h, int(option), int(value)))
+
+    def get_option(self, option):
+        v = ctypes.c_longlong(0)
+        check(lib().ske_batch_get_option(self._h, int(option), ctypes.byref(v)))
+        return v.value
 
     def profile(self, name):
         ms = ctypes.c_double(0)

@@ -33,11 +33,13 @@ def _terrains(n):
     return {k: np.ascontiguousarray(v, dtype=np.float32) for k, v in out.items()}
 
 
+@pytest.mark.parametrize("group", [1, 4])
 @pytest.mark.parametrize("name", ["cone", "walls", "pit", "spikes", "steps", "fbm_plus_cone"])
-def test_exit_terrains_bit_exact_vs_oracle(name):
+def test_exit_terrains_bit_exact_vs_oracle(name, group):
     grid = _terrains(160)[name]
-    print("terrain", name, flush=True)
+    print("terrain", name, "group", group, flush=True)
     with ske.SkewEngine(grid) as e:
+        e.set_option(_lib.OPT_SDEM_GROUP, group)
         out = e.run(ske.KERNEL_SDEM, povh=ske.POVH_DEFAULT)
     ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=1, povh=ske.POVH_DEFAULT)
     assert np.array_equal(out, ref), name
@@ -50,20 +52,26 @@ def test_exit_large_dem_matches_per_step_kernel(n, hurst):
     ranges = [(0, 2), (44, 47), (89, 91), (134, 136), (178, 180)]
     with ske.SkewEngine(grid) as e:
         fast = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
+        e.set_option(_lib.OPT_SDEM_GROUP, 4)
+        packed = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
         e.set_option(_lib.OPT_SDEM_VARIANT, 2)
         full = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
-    for (s0, s1), a, b in zip(ranges, fast, full):
-        assert np.array_equal(a, b), (n, s0, s1)
+    for (s0, s1), a, b, c in zip(ranges, fast, packed, full):
+        assert np.array_equal(a, c), (n, s0, s1)
+        assert np.array_equal(b, c), (n, s0, s1, "group 4")
 
 
-@pytest.mark.parametrize("waves,pers,occ", [(4, 0, 0), (4, 8, 8), (2, 16, 8), (8, 4, 0)])
-def test_exit_batch_launch_shapes_match_per_step_kernel(waves, pers, occ):
+@pytest.mark.parametrize("waves,pers,occ,group", [(4, 0, 0, 1), (4, 8, 8, 1), (2, 16, 8, 1), (8, 4, 0, 1),
+                                                  (4, 0, 0, 4), (4, 8, 8, 4), (2, 16, 8, 4), (8, 4, 0, 4),
+                                                  (4, 16, 8, 4)])
+def test_exit_batch_launch_shapes_match_per_step_kernel(waves, pers, occ, group):
     dems = [synth.dem(n, 0.75, 30 + i, 3000.0) for i, n in enumerate((1000, 1500, 700))]
-    print("batch shape", waves, pers, occ, flush=True)
+    print("batch shape", waves, pers, occ, group, flush=True)
     with ske.SkewBatch(dems) as b:
         b.set_option(_lib.OPT_SDEM_WAVES, waves)
         b.set_option(_lib.OPT_SDEM_PERSISTENT, pers)
         b.set_option(_lib.OPT_SDEM_OCCUPANCY, occ)
+        b.set_option(_lib.OPT_SDEM_GROUP, group)
         b.run(ske.KERNEL_SDEM)
         outs = b.outputs()
     for g, o in zip(dems, outs):
@@ -73,10 +81,12 @@ def test_exit_batch_launch_shapes_match_per_step_kernel(waves, pers, occ):
         assert np.array_equal(o, ref)
 
 
-def test_step_counter_counts_without_changing_results():
+@pytest.mark.parametrize("group", [1, 4])
+def test_step_counter_counts_without_changing_results(group):
     grid = synth.dem(500, 0.8, 1, 1500.0)
     _, steps = ske.work(500, 500)
     with ske.SkewEngine(grid) as e:
+        e.set_option(_lib.OPT_SDEM_GROUP, group)
         plain = e.run(ske.KERNEL_SDEM)
         assert e.counter("op_sdem_lane_steps") == 0.0  # not counting
         e.set_option(_lib.OPT_COUNT_STEPS, 1)
@@ -89,3 +99,17 @@ def test_step_counter_counts_without_changing_results():
     assert np.array_equal(plain, counted)
     assert 0 < n1 < 1.5 * steps and n1 % 2048 == 0  # whole 32-step blocks of 64 lanes
     assert n2 == 2 * n1 and n0 == 0.0
+
+
+def test_group_option_round_trip_and_errors():
+    grid = synth.dem(64, 0.8, 2, 1500.0)
+    with ske.SkewEngine(grid) as e:
+        assert e.get_option(_lib.OPT_SDEM_GROUP) == 4  # default: lane compaction
+        e.set_option(_lib.OPT_SDEM_GROUP, 1)
+        assert e.get_option(_lib.OPT_SDEM_GROUP) == 1
+        with pytest.raises(_lib.SkeError):
+            e.set_option(_lib.OPT_SDEM_GROUP, 3)
+        assert e.get_option(_lib.OPT_SDEM_GROUP) == 1
+        out = e.run(ske.KERNEL_SDEM)
+    ref, _ = oracle.execute(oracle.SDEM, grid, nthreads=1)
+    assert np.array_equal(out, ref)
