@@ -252,6 +252,7 @@ def extra_c5_learned(batch=64):
     bins, Poisson noise at I0 = 1e5, batch 64: time of plain Ram-Lak FBP vs
     the learned reconstruction (filter + back-projection + CNN) on the same
     sinograms, and held-out PSNR of both with the committed parameters."""
+    from skewengine_amd import _lib
     from skewengine_amd import learned as lm
     from skewengine_amd.fbp import FBP
     path = lm.default_weights(512, 60, 725)
@@ -269,7 +270,8 @@ def extra_c5_learned(batch=64):
         f.synchronize()
         cnn_ms = f.profile("cnn_forward")[0]
         f.set_profile(False)
-        flop = 2 * 9 * (1 * 32 + 32 * 32 + 32 * 32 + 32 * 1) * 512 * 512 * batch
+        c = _lib.LEARNED_CHANNELS
+        flop = 2 * 9 * (1 * c + c * c + c * c + c * 1) * 512 * 512 * batch
         res.update({"fbp_ms_per_slice": t_fbp * 1e3 / batch, "learned_ms_per_slice": t_l * 1e3 / batch,
                     "time_ratio": t_l / t_fbp, "cnn_ms_per_slice": cnn_ms / batch,
                     "cnn_TFLOPs": flop / (cnn_ms / 1e3) / 1e12})

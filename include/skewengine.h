@@ -240,16 +240,16 @@ int ske_result_pngs(const char *dir, int kernel_id, int width, int height, const
  *   r0  = BP(h * p), h[m] = taps[|m|] (n_det symmetric taps; the DCT of the
  *         taps is the filter's per-frequency response)
  *   out = r0 + conv(relu(conv(relu(conv(relu(conv(r0))))))) with 3x3 convs
- *         1->32->32->32->1, zero padding, bf16 weights and activations,
+ *         1->16->16->16->1, zero padding, bf16 weights and activations,
  *         fp32 accumulation.
- * Parameters (fp32, SKE_LEARNED_TAPS + n_det of them): W1[32][9] b1[32]
- * W2[32][9][32] b2[32] W3[32][9][32] b3[32] W4[9][32] b4 taps[n_det]; tap
+ * Parameters (fp32, SKE_LEARNED_TAPS + n_det of them): W1[16][9] b1[16]
+ * W2[16][9][16] b2[16] W3[16][9][16] b3[16] W4[9][16] b4 taps[n_det]; tap
  * t = 3*ky + kx reads pixel (i+ky-1, j+kx-1).  The image side must be a
  * multiple of 32. */
 typedef struct ske_learned ske_learned;
-#define SKE_LEARNED_TAPS 19105 /* offset of the filter taps in the parameters */
+#define SKE_LEARNED_TAPS 4945  /* offset of the filter taps in the parameters */
 #define SKE_LEARNED_R0 0       /* fp32 batch x n x n: learned-filter FBP */
-#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 32: hidden activations of the last
+#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 16: hidden activations of the last
                                 * ske_learned_gradient / train_step (reconstruct keeps them in LDS) */
 #define SKE_LEARNED_A2 2
 #define SKE_LEARNED_A3 3

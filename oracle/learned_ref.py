@@ -6,17 +6,17 @@ CNN).  This is a float64 numpy restatement of the model the GPU kernels
 compute, forward and backward:
 
   r0  = BP(h * p),  h[m] = taps[|m|]                       (std_fbp geometry)
-  a1  = bf(relu(conv(bf(r0); bf(W1)) + b1))               1 -> 32
-  a2  = bf(relu(conv(a1; bf(W2)) + b2))                   32 -> 32
-  a3  = bf(relu(conv(a2; bf(W3)) + b3))                   32 -> 32
-  out = r0 + conv(a3; bf(W4)) + b4                        32 -> 1
+  a1  = bf(relu(conv(bf(r0); bf(W1)) + b1))               1 -> 16
+  a2  = bf(relu(conv(a1; bf(W2)) + b2))                   16 -> 16
+  a3  = bf(relu(conv(a2; bf(W3)) + b3))                   16 -> 16
+  out = r0 + conv(a3; bf(W4)) + b4                        16 -> 1
   loss = mean((out - target)^2)
 
 bf() rounds to bfloat16 (nearest even) when `bf16=True`, the identity
 otherwise; gradients pass straight through bf() (the GPU differentiates the
 same way).  3x3 convolutions with zero padding, tap t = 3*ky + kx reading
-pixel (i+ky-1, j+kx-1).  Parameter vector layout: W1[32][9] b1[32]
-W2[32][9][32] b2[32] W3[32][9][32] b3[32] W4[9][32] b4 taps[n_det].
+pixel (i+ky-1, j+kx-1).  Parameter vector layout: W1[16][9] b1[16]
+W2[16][9][16] b2[16] W3[16][9][16] b3[16] W4[9][16] b4 taps[n_det].
 
 Pinned by tests/test_oracle_learned.py: central finite differences of the
 FP64 loss (bf16 off) for every parameter block and for r0, the adjoint
@@ -27,14 +27,14 @@ import numpy as np
 
 from . import std_fbp
 
-C = 32
+C = 16
 OFF = {}
 _o = 0
 for _name, _size in (("W1", C * 9), ("b1", C), ("W2", C * 9 * C), ("b2", C), ("W3", C * 9 * C), ("b3", C),
                      ("W4", 9 * C), ("b4", 1)):
     OFF[_name] = (_o, _size)
     _o += _size
-TAPS = _o  # 19105
+TAPS = _o  # 4945
 
 
 def bf16(x):

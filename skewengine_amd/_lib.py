@@ -16,7 +16,8 @@ OPT_SDEM_OCCUPANCY = 7
 OPT_RADON_LOCAL = 8
 OPT_COUNT_STEPS = 9
 OPT_SDEM_GROUP = 10
-LEARNED_TAPS = 19105
+LEARNED_TAPS = 4945
+LEARNED_CHANNELS = 16
 LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180
 

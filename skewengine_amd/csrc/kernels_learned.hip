@@ -3,16 +3,16 @@
 // CNN on the reconstruction:
 //
 //   r0 = BP(filter_h(p))                      (fp32, n x n)
-//   a1 = relu(conv3x3(r0;  1->32) + b1)       (bf16, NHWC)
-//   a2 = relu(conv3x3(a1; 32->32) + b2)
-//   a3 = relu(conv3x3(a2; 32->32) + b3)
-//   out = r0 + conv3x3(a3; 32->1) + b4        (fp32)
+//   a1 = relu(conv3x3(r0;  1->16) + b1)       (bf16, NHWC)
+//   a2 = relu(conv3x3(a1; 16->16) + b2)
+//   a3 = relu(conv3x3(a2; 16->16) + b3)
+//   out = r0 + conv3x3(a3; 16->1) + b4        (fp32)
 //
 // Weights are kept in fp32 and rounded to bf16 where they are read
 // (bf16 weights); activations a1..a3 are stored in bf16; every sum
 // accumulates in fp32.  Convolutions use zero padding; tap t = 3*ky + kx
 // reads input pixel (i + ky - 1, j + kx - 1).  Weight layouts: W1[co][t],
-// W2/W3[co][t][ci], W4[t][ci].  Image sides are a multiple of 32.
+// W2/W3[co][t][ci], W4[t][ci].  Image sides are a multiple of 16.
 //
 // Training kernels: the exact gradients of the loss mean((out - target)^2)
 // with respect to every weight, the filter taps (through the adjoint of the
@@ -63,348 +63,232 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
     }
 }
 
-// The convolutions with 32 output channels run on the matrix cores as
-// implicit GEMMs, D[co][px] = sum_k W[co][k] X[k][px] over 32-pixel groups,
-// with v_mfma_f32_32x32x16_bf16 (fp32 accumulate, the bias as the initial
-// accumulator).  Operand lane layout: A[i][k] at lane i + 32*(k/8), element
-// k%8; B[k][j] at lane j + 32*(k/8); D[i][j] at lane j + 32*((i/4)%2),
-// register 4*(i/8) + i%4.
-//   conv 1 -> 32: k = tap (9 taps padded to 16), one MFMA per group;
-//   conv 32 -> 32: k = (tap, 16 channels), 18 MFMAs per group, taps in
-//   order, each with channels 0-15 then 16-31.
-// Activation tiles sit in LDS as bf16 with an 80-byte pixel pitch (32
-// channels + 8 pad: conflict-free 16-byte reads).
+// The whole CNN forward, fused: one workgroup per 16 x 16 output tile
+// computes a1 on the 22 x 22 region around it (from a 24 x 24 tile of r0),
+// a2 on 20 x 20, a3 on 18 x 18, then out -- all in LDS, so HBM sees r0 once
+// and out once (and, when training, the centre 16 x 16 of a1, a2 and a3,
+// which the backward kernels read).  Region pixels outside the image are
+// zero, the zero padding of the next layer.  Every value is computed by the
+// same instruction sequence whichever tile or group it falls in, so the
+// training forward and the inference forward agree bit for bit.
+//
+// Matrix cores: v_mfma_f32_16x16x32_bf16 computes D[co][px] += sum_k
+// W[co][k] X[k][px] for a group of 16 pixels (fp32 accumulate, the bias as
+// the initial accumulator).  Lane layout: A[i][k] at lane i + 16*(k/8),
+// element k%8; B[k][j] at lane j + 16*(k/8), element k%8; D[i][j] at lane
+// j + 16*(i/4), register i%4.
+//   conv 1 -> 16: k = tap (9 taps, then zeros): one MFMA per group;
+//   conv 16 -> 16: k = (tap, channel) over the tap pairs (0,1), (2,3),
+//   (4,5), (6,7), (8,-): five MFMAs per group, taps in order.
+//   conv 16 -> 1: v_dot2c_f32_bf16 on channel pairs, channels 0-7 and 8-15
+//   in two partial sums (taps in order, pairs in order), then (s0+s1) + b4.
+// Activation tiles are bf16 NHWC in LDS with a 48-byte pixel pitch (16
+// channels + 8 pad), so the 16-byte operand reads of 8 consecutive lanes
+// fall in distinct banks.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-constexpr int kMR = 8;        // tile rows of the unfused kernels
-constexpr int kMW = 32;       // tile columns
-constexpr int kMP = 40;       // LDS pixel pitch in bf16 (32 channels + 8 pad)
-constexpr int kMHalo = (kMR + 2) * (kMW + 2);
-
-__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
-
-// The 8 bf16 B-operand entries of conv 1 -> 32 for one pixel: taps 8*hi ..
-// 8*hi+7 of the bf16 image tile `s0` (row pitch w0) around (y, x).
-__device__ __forceinline__ bf16x8 conv_in_operand(const uint16_t* s0, int w0, int y, int x, int hi) {
-    uint32_t e[8];
-    if (hi == 0) {
-#pragma unroll
-        for (int t = 0; t < 8; t++) e[t] = s0[(y + t / 3) * w0 + x + t % 3];
-    } else {
-        e[0] = s0[(y + 2) * w0 + x + 2];
-#pragma unroll
-        for (int t = 1; t < 8; t++) e[t] = 0;
-    }
-    return as_bf16x8(make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16)));
-}
-
-// relu, bf16, and the group's 16 values per lane into an NHWC tile (8-byte
-// stores of 4 consecutive channels); `keep` false writes zeros.
-__device__ __forceinline__ void store_group(uint16_t* dst, const f32x16& acc, bool keep) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        uint2 v;
-        v.x = pack_bf16(fmaxf(acc[4 * q], 0.0f), fmaxf(acc[4 * q + 1], 0.0f));
-        v.y = pack_bf16(fmaxf(acc[4 * q + 2], 0.0f), fmaxf(acc[4 * q + 3], 0.0f));
-        if (!keep) v = make_uint2(0, 0);
-        *(uint2*)(dst + 8 * q) = v;
-    }
-}
-
-// conv 1 -> 32 + bias + relu.  Input r0 fp32, rounded to bf16 as it is
-// staged.  Workgroup: 4 waves over an 8 x 32 pixel tile, two rows each.
-__global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ r0, const uint16_t* __restrict__ w1b,
-                                                     const float* __restrict__ prm, uint16_t* __restrict__ a1, int n) {
-    __shared__ uint16_t s0[kMHalo];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * kMR, j0 = blockIdx.x * kMW;
-    const float* img = r0 + (size_t)b * n * n;
-    for (int t = threadIdx.x; t < kMHalo; t += 256) {
-        const int ii = i0 + t / (kMW + 2) - 1, jj = j0 + t % (kMW + 2) - 1;
-        s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l32 = lane & 31, hi = lane >> 5;
-    const bf16x8 w = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
-    float bias[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) bias[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < kMR / 4; rr++) {
-        const int row = wave * (kMR / 4) + rr;
-        f32x16 acc;
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = bias[r];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, conv_in_operand(s0, kMW + 2, row, l32, hi), acc, 0, 0, 0);
-        store_group(a1 + (((size_t)b * n + i0 + row) * n + j0 + l32) * C + 4 * hi, acc, true);
-    }
-}
-
-// conv 32 -> 32 + bias + relu, bf16 NHWC in/out.  Workgroup: 4 waves over
-// an 8 x 32 pixel tile, each wave two rows; the (8+2) x (32+2) halo in LDS;
-// each wave keeps the 18 weight operands of its 32 output channels in
-// registers.
-__global__ __launch_bounds__(256) void k_conv_mid_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wb,
-                                                      const float* __restrict__ prm, int boff,
-                                                      uint16_t* __restrict__ y, int n) {
-    __shared__ __align__(16) uint16_t xs[kMHalo * kMP];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * kMR, j0 = blockIdx.x * kMW;
-    const uint16_t* xb = x + (size_t)b * n * n * C;
-    for (int t = threadIdx.x; t < kMHalo * 4; t += 256) {
-        const int pix = t >> 2, part = t & 3;
-        const int ii = i0 + pix / (kMW + 2) - 1, jj = j0 + pix % (kMW + 2) - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (ii >= 0 && ii < n && jj >= 0 && jj < n) v = *(const uint4*)(xb + ((size_t)ii * n + jj) * C + 8 * part);
-        *(uint4*)(xs + pix * kMP + 8 * part) = v;
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l32 = lane & 31, hi = lane >> 5;
-    bf16x8 w[18];
-#pragma unroll
-    for (int t = 0; t < 9; t++)
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            w[2 * t + h] = as_bf16x8(*(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi));
-        }
-    float bias[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) bias[r] = prm[boff + 8 * (r >> 2) + 4 * hi + (r & 3)];
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < kMR / 4; rr++) {
-        const int row = wave * (kMR / 4) + rr;
-        f32x16 acc;
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = bias[r];
-#pragma unroll
-        for (int t = 0; t < 9; t++) {
-            const uint16_t* xp = xs + ((row + t / 3) * (kMW + 2) + l32 + t % 3) * kMP + 8 * hi;
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], as_bf16x8(*(const uint4*)(xp + 16 * h)),
-                                                              acc, 0, 0, 0);
-            }
-        }
-        store_group(y + (((size_t)b * n + i0 + row) * n + j0 + l32) * C + 4 * hi, acc, true);
-    }
-}
-
-// conv 32 -> 1 + bias, residual add; optionally the loss and its gradient.
-// The a3 halo stays bf16 in LDS with a 17-word pixel pitch, so the lanes
-// (consecutive pixels) read distinct banks.  Channel pairs are summed with
-// v_dot2c_f32_bf16 (weights rounded to bf16) in two partial sums, channels
-// 0-15 and 16-31 (taps in order, pairs in order within a tap), then
-// (s0 + s1) + b4.
-constexpr int kOP = C / 2 + 1;  // LDS pixel pitch in 32-bit words
-__device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
-    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
-}
-
-__global__ __launch_bounds__(256) void k_conv_out_fwd(const uint16_t* __restrict__ a3, const float* __restrict__ r0,
-                                                      const float* __restrict__ prm, float* __restrict__ out,
-                                                      const float* __restrict__ target, float* __restrict__ dout,
-                                                      float inv_count, double* __restrict__ loss, int n) {
-    __shared__ uint32_t w[9 * C / 2];
-    __shared__ uint32_t xs[(T + 2) * (T + 2) * kOP];
-    __shared__ float red[256];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    for (int t = threadIdx.x; t < 9 * C / 2; t += 256) w[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
-    const float b4 = prm[kB4];
-    const uint16_t* xb = a3 + (size_t)b * n * n * C;
-    for (int t = threadIdx.x; t < (T + 2) * (T + 2) * 4; t += 256) {
-        const int pix = t >> 2, part = t & 3;
-        const int ii = i0 + pix / (T + 2) - 1, jj = j0 + pix % (T + 2) - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (ii >= 0 && ii < n && jj >= 0 && jj < n) v = *(const uint4*)(xb + ((size_t)ii * n + jj) * C + 8 * part);
-        uint32_t* d = xs + pix * kOP + 4 * part;
-        d[0] = v.x;
-        d[1] = v.y;
-        d[2] = v.z;
-        d[3] = v.w;
-    }
-    __syncthreads();
-    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float s0 = 0.0f, s1 = 0.0f;  // channels 0-15 and 16-31
-    for (int tap = 0; tap < 9; tap++) {
-        const uint32_t* xp = xs + ((ty + tap / 3) * (T + 2) + tx + tap % 3) * kOP;
-#pragma unroll
-        for (int c2 = 0; c2 < C / 4; c2++) s0 = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s0);
-#pragma unroll
-        for (int c2 = C / 4; c2 < C / 2; c2++) s1 = dot2_bf16(w[tap * (C / 2) + c2], xp[c2], s1);
-    }
-    const size_t p = ((size_t)b * n + i0 + ty) * n + j0 + tx;
-    const float o = r0[p] + ((s0 + s1) + b4);
-    out[p] = o;
-    if (target) {
-        const float d = o - target[p];
-        dout[p] = 2.0f * d * inv_count;
-        red[threadIdx.x] = d * d;
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
-    }
-}
-
-// The whole CNN forward for inference, fused: one workgroup per 16 x 16
-// output tile computes a1 on the 22 x 22 region around it (from a 24 x 24
-// tile of r0), a2 on 20 x 20, a3 on 18 x 18, then out -- all in LDS, so HBM
-// sees r0 once and out once.  Region pixels outside the image are zero,
-// the zero padding of the next layer.  The arithmetic per value is the
-// unfused kernels' (same MFMA sequence, same FMA order for conv 32 -> 1),
-// so the output is bit-identical to theirs.  8 waves; each layer's pixels
-// go in 32-pixel groups, group g to wave g mod 8.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+static_assert(C == 16, "the fused forward is written for 16 hidden channels");
+constexpr int kP = 24;           // LDS pixel pitch in bf16
 constexpr int kFT = 16;          // output tile side
 constexpr int kF0 = kFT + 8;     // r0 region side
 constexpr int kF1 = kFT + 6;     // a1
 constexpr int kF2 = kFT + 4;     // a2
 constexpr int kF3 = kFT + 2;     // a3
+constexpr int kFWaves = 4;
 
-// One conv 32 -> 32 layer from the LDS region `in` (side wi) to `out` (side
-// wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
-// groups g and g + 8 together (two independent accumulators), so the loads
-// of one group overlap the matrix-core work of the other; the bias (32
-// floats in LDS) seeds both accumulators.
-__device__ __forceinline__ void fused_acc_init(f32x16& acc, const float* bias, int hi) {
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
+}
+
+// B operand of conv 1 -> 16 for the a1-region pixel (y, x): taps 8*kb ..
+// 8*kb+7 of the bf16 r0 tile (zeros past tap 8).
+__device__ __forceinline__ bf16x8 in_operand(const uint16_t* s0, int y, int x, int kb) {
+    uint32_t e[8];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const float4 b = *(const float4*)(bias + 8 * q + 4 * hi);
-        acc[4 * q] = b.x;
-        acc[4 * q + 1] = b.y;
-        acc[4 * q + 2] = b.z;
-        acc[4 * q + 3] = b.w;
+    for (int t = 0; t < 8; t++) e[t] = 0;
+    if (kb == 0) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) e[t] = s0[(y + t / 3) * kF0 + x + t % 3];
+    } else if (kb == 1) {
+        e[0] = s0[(y + 2) * kF0 + x + 2];
+    }
+    return as_bf16x8(make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16)));
+}
+
+// relu, bf16, and a lane's 4 channels of one pixel (8 bytes); `keep` false
+// writes zeros.
+__device__ __forceinline__ void store4(uint16_t* dst, const f32x4& acc, bool keep) {
+    uint2 v;
+    v.x = pack_bf16(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f));
+    v.y = pack_bf16(fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+    if (!keep) v = make_uint2(0, 0);
+    *(uint2*)dst = v;
+}
+
+// A operands of a conv 16 -> 16 layer: lane (co = l%16, kb = l/16) holds
+// W[co][2tp + kb/2][8(kb%2) .. 8(kb%2)+7] for tap pair tp (zero for tap 9).
+__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[5], const uint16_t* __restrict__ wb, int co, int kb) {
+#pragma unroll
+    for (int tp = 0; tp < 5; tp++) {
+        const int tap = 2 * tp + (kb >> 1);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (tap < 9) v = *(const uint4*)(wb + (co * 9 + tap) * C + 8 * (kb & 1));
+        w[tp] = as_bf16x8(v);
     }
 }
 
-__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[18], const uint16_t* __restrict__ wb, int l32, int hi) {
-#pragma unroll
-    for (int t = 0; t < 9; t++)
-#pragma unroll
-        for (int h = 0; h < 2; h++) w[2 * t + h] = as_bf16x8(*(const uint4*)(wb + (l32 * 9 + t) * C + 16 * h + 8 * hi));
-}
-
-__device__ __forceinline__ void fused_mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
-                                                const bf16x8 (&w)[18], const float* bias, int wave, int l32,
-                                                int hi) {
+// One conv 16 -> 16 layer from the LDS region `in` (side wi) to `out` (side
+// wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
+// groups g and g + kFWaves together (two independent accumulators).  Lanes
+// of the absent tap 9 read tap 8 again; their weights are zero.
+__device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
+                                          const bf16x8 (&w)[5], const f32x4 bias, int wave, int j16, int kb) {
     const int wo = wi - 2, count = wo * wo;
-    for (int g0 = wave; g0 * 32 < count; g0 += 16) {
-        const int g1 = g0 + 8;
-        const int p0 = g0 * 32 + l32, p1 = g1 * 32 + l32;
+    int toff[5];
+#pragma unroll
+    for (int tp = 0; tp < 5; tp++) {
+        const int tap = min(2 * tp + (kb >> 1), 8);
+        toff[tp] = ((tap / 3) * wi + tap % 3) * kP + 8 * (kb & 1);
+    }
+    for (int g0 = wave; g0 * 16 < count; g0 += 2 * kFWaves) {
+        const int g1 = g0 + kFWaves;
+        const int p0 = g0 * 16 + j16, p1 = g1 * 16 + j16;
         const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
         const int y0 = q0 / wo, x0 = q0 % wo, y1 = q1 / wo, x1 = q1 % wo;
-        const uint16_t* b0 = in + (y0 * wi + x0) * kMP + 8 * hi;
-        f32x16 acc0;
-        fused_acc_init(acc0, bias, hi);
-        if (g1 * 32 < count) {  // two groups (wave-uniform branch)
-            const uint16_t* b1 = in + (y1 * wi + x1) * kMP + 8 * hi;
-            f32x16 acc1;
-            fused_acc_init(acc1, bias, hi);
+        const uint16_t* b0 = in + (y0 * wi + x0) * kP;
+        f32x4 acc0 = bias;
+        if (g1 * 16 < count) {  // two groups (wave-uniform branch)
+            const uint16_t* b1 = in + (y1 * wi + x1) * kP;
+            f32x4 acc1 = bias;
 #pragma unroll
-            for (int t = 0; t < 9; t++) {
-                const int o = ((t / 3) * wi + t % 3) * kMP;
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + o + 16 * h));
-                    const bf16x8 v1 = as_bf16x8(*(const uint4*)(b1 + o + 16 * h));
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], v0, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h], v1, acc1, 0, 0, 0);
-                }
+            for (int tp = 0; tp < 5; tp++) {
+                const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + toff[tp]));
+                const bf16x8 v1 = as_bf16x8(*(const uint4*)(b1 + toff[tp]));
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], v0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], v1, acc1, 0, 0, 0);
             }
             const int iy = oy + y1, ix = ox + x1;
-            if (p1 < count) store_group(out + p1 * kMP + 4 * hi, acc1, iy >= 0 && iy < n && ix >= 0 && ix < n);
+            if (p1 < count) store4(out + p1 * kP + 4 * kb, acc1, iy >= 0 && iy < n && ix >= 0 && ix < n);
         } else {
 #pragma unroll
-            for (int t = 0; t < 9; t++) {
-                const int o = ((t / 3) * wi + t % 3) * kMP;
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[2 * t + h],
-                                                                   as_bf16x8(*(const uint4*)(b0 + o + 16 * h)), acc0,
-                                                                   0, 0, 0);
-            }
+            for (int tp = 0; tp < 5; tp++)
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], as_bf16x8(*(const uint4*)(b0 + toff[tp])), acc0,
+                                                               0, 0, 0);
         }
         const int iy = oy + y0, ix = ox + x0;
-        if (p0 < count) store_group(out + p0 * kMP + 4 * hi, acc0, iy >= 0 && iy < n && ix >= 0 && ix < n);
+        if (p0 < count) store4(out + p0 * kP + 4 * kb, acc0, iy >= 0 && iy < n && ix >= 0 && ix < n);
     }
 }
 
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_cnn_fused_fwd(const float* __restrict__ r0, const float* __restrict__ prm,
-                                                       const uint16_t* __restrict__ w1b,
-                                                       const uint16_t* __restrict__ wb2,
-                                                       const uint16_t* __restrict__ wb3, float* __restrict__ out,
-                                                       int n) {
-    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kMP];  // a1, then a3
-    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kMP];  // a2
-    __shared__ uint16_t s0[kF0 * kF0];                     // r0 as bf16
-    __shared__ uint32_t w4[9 * C / 2];                     // W4 as bf16 pairs
-    __shared__ __align__(16) float sbias[2 * C];            // b2, b3
+// Copy the centre 16 x 16 pixels of an LDS region (side wi, centre offset
+// `off`) to the NHWC image `dst` (training: the backward kernels read it).
+__device__ __forceinline__ void store_centre(const uint16_t* reg, int wi, int off, uint16_t* __restrict__ dst,
+                                             int i0, int j0, int n) {
+    for (int t = threadIdx.x; t < kFT * kFT * 2; t += 64 * kFWaves) {
+        const int px = t >> 1, h = t & 1;
+        const int y = px / kFT, x = px % kFT;
+        const uint4 v = *(const uint4*)(reg + ((y + off) * wi + x + off) * kP + 8 * h);
+        *(uint4*)(dst + ((size_t)(i0 + y) * n + j0 + x) * C + 8 * h) = v;
+    }
+}
+
+template <bool kTrain>
+__global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __restrict__ r0,
+                                                                const float* __restrict__ prm,
+                                                                const uint16_t* __restrict__ w1b,
+                                                                const uint16_t* __restrict__ wb2,
+                                                                const uint16_t* __restrict__ wb3,
+                                                                float* __restrict__ out, int n,
+                                                                uint16_t* __restrict__ a1g, uint16_t* __restrict__ a2g,
+                                                                uint16_t* __restrict__ a3g,
+                                                                const float* __restrict__ target,
+                                                                float* __restrict__ dout, float inv_count,
+                                                                double* __restrict__ loss) {
+    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kP];  // a1, then a3
+    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kP];  // a2
+    __shared__ uint16_t s0[kF0 * kF0];                    // r0 as bf16
+    __shared__ uint32_t w4[9 * C / 2];                    // W4 as bf16 pairs
+    __shared__ float red[64 * kFWaves];
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l32 = lane & 31, hi = lane >> 5;
-    // all global loads of the prologue in flight together: W2 operands, W1,
-    // biases, W4 and the r0 tile
-    bf16x8 w[18];
-    load_mid_weights(w, wb2, l32, hi);
-    const bf16x8 w1 = as_bf16x8(*(const uint4*)(w1b + l32 * 16 + 8 * hi));
-    float bias1[16];
+    const int j16 = lane & 15, kb = lane >> 4;  // A row / B column, and k block
+    // prologue loads in flight together: W2 operands, W1, biases, W4, r0 tile
+    bf16x8 w[5];
+    load_mid_weights(w, wb2, j16, kb);
+    const bf16x8 w1 = as_bf16x8(kb < 2 ? *(const uint4*)(w1b + j16 * 16 + 8 * kb) : make_uint4(0, 0, 0, 0));
+    f32x4 bias1, bias2, bias3;
 #pragma unroll
-    for (int r = 0; r < 16; r++) bias1[r] = prm[kB1 + 8 * (r >> 2) + 4 * hi + (r & 3)];
-    for (int t = threadIdx.x; t < kF0 * kF0; t += 512) {
+    for (int r = 0; r < 4; r++) {
+        bias1[r] = prm[kB1 + 4 * kb + r];
+        bias2[r] = prm[kB2 + 4 * kb + r];
+        bias3[r] = prm[kB3 + 4 * kb + r];
+    }
+    for (int t = threadIdx.x; t < kF0 * kF0; t += 64 * kFWaves) {
         const int ii = i0 - 4 + t / kF0, jj = j0 - 4 + t % kF0;
         s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
     }
-    for (int t = threadIdx.x; t < 9 * C / 2; t += 512) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
-    if (threadIdx.x < 2 * C) sbias[threadIdx.x] = prm[(threadIdx.x < C ? kB2 : kB3 - C) + threadIdx.x];
+    for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
     __syncthreads();
     {   // a1 on the 22 x 22 region at (i0-3, j0-3)
         const int count = kF1 * kF1;
-        for (int g = wave; g * 32 < count; g += 8) {
-            const int p = g * 32 + l32;
+        for (int g = wave; g * 16 < count; g += kFWaves) {
+            const int p = g * 16 + j16;
             const int pc = min(p, count - 1);
             const int y = pc / kF1, x = pc % kF1;
-            f32x16 acc;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = bias1[r];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, conv_in_operand(s0, kF0, y, x, hi), acc, 0, 0, 0);
+            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, in_operand(s0, y, x, kb), bias1, 0, 0, 0);
             const int iy = i0 - 3 + y, ix = j0 - 3 + x;
-            if (p < count) store_group(sa + p * kMP + 4 * hi, acc, iy >= 0 && iy < n && ix >= 0 && ix < n);
+            if (p < count) store4(sa + p * kP + 4 * kb, acc, iy >= 0 && iy < n && ix >= 0 && ix < n);
         }
     }
     __syncthreads();
-    fused_mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, w, sbias, wave, l32, hi);
-    load_mid_weights(w, wb3, l32, hi);  // in flight while the other waves finish
+    const size_t img_off = (size_t)b * n * n * C;
+    if (kTrain) store_centre(sa, kF1, 3, a1g + img_off, i0, j0, n);
+    mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
+    load_mid_weights(w, wb3, j16, kb);  // in flight while the other waves finish
     __syncthreads();
-    fused_mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, w, sbias + C, wave, l32, hi);
+    if (kTrain) store_centre(sb, kF2, 2, a2g + img_off, i0, j0, n);
+    mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
-    {   // conv 32 -> 1 + bias + residual: two threads per pixel (channels
-        // 0-15 and 16-31), combined as in k_conv_out_fwd
-        const int px = threadIdx.x >> 1, part = threadIdx.x & 1;
+    if (kTrain) store_centre(sa, kF3, 1, a3g + img_off, i0, j0, n);
+    // conv 16 -> 1 + bias + residual: one thread per pixel and channel half
+    float dsum = 0.0f;  // training: this thread's squared errors
+    for (int t = threadIdx.x; t < 2 * kFT * kFT; t += 64 * kFWaves) {
+        const int px = t >> 1, part = t & 1;
         const int y = px / kFT, x = px % kFT;
         float acc = 0.0f;
-        for (int tap = 0; tap < 9; tap++) {
-            const uint16_t* xp = sa + ((y + tap / 3) * kF3 + x + tap % 3) * kMP + 16 * part;
 #pragma unroll
-            for (int c8 = 0; c8 < 2; c8++) {
-                const uint4 v = *(const uint4*)(xp + 8 * c8);
-                const uint32_t* wp = w4 + tap * (C / 2) + 8 * part + 4 * c8;
-                acc = dot2_bf16(wp[0], v.x, acc);
-                acc = dot2_bf16(wp[1], v.y, acc);
-                acc = dot2_bf16(wp[2], v.z, acc);
-                acc = dot2_bf16(wp[3], v.w, acc);
-            }
+        for (int tap = 0; tap < 9; tap++) {
+            const uint4 v = *(const uint4*)(sa + ((y + tap / 3) * kF3 + x + tap % 3) * kP + 8 * part);
+            const uint32_t* wp = w4 + tap * (C / 2) + 4 * part;
+            acc = dot2_bf16(wp[0], v.x, acc);
+            acc = dot2_bf16(wp[1], v.y, acc);
+            acc = dot2_bf16(wp[2], v.z, acc);
+            acc = dot2_bf16(wp[3], v.w, acc);
         }
         const float other = __shfl_xor(acc, 1, 64);
+        const float s0v = part == 0 ? acc : other, s1v = part == 0 ? other : acc;
+        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
+        const float o = img[(size_t)(i0 + y) * n + j0 + x] + ((s0v + s1v) + prm[kB4]);
         if (part == 0) {
-            const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
-            out[p] = img[(size_t)(i0 + y) * n + j0 + x] + ((acc + other) + prm[kB4]);
+            out[p] = o;
+            if (kTrain && target) {
+                const float d = o - target[p];
+                dout[p] = 2.0f * d * inv_count;
+                dsum += d * d;
+            }
         }
+    }
+    if (kTrain && target) {  // uniform
+        red[threadIdx.x] = dsum;
+        __syncthreads();
+        for (int st = 32 * kFWaves; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
     }
 }
 
@@ -739,23 +623,21 @@ hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, u
     return hipGetLastError();
 }
 
-// Inference: the fused forward (n a multiple of 16).
+// Inference: the fused forward, no activations stored (n a multiple of 16).
 hipError_t launch_cnn_infer(const LearnedBufs& B, const float* prm, int n, int batch, hipStream_t s) {
-    hipLaunchKernelGGL(k_cnn_fused_fwd, dim3(n / kFT, n / kFT, batch), dim3(512), 0, s, B.r0, prm, B.w1b, B.wb2, B.wb3,
-                       B.out, n);
+    hipLaunchKernelGGL(k_cnn_fused_fwd<false>, dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0, prm,
+                       B.w1b, B.wb2, B.wb3, B.out, n, (uint16_t*)nullptr, (uint16_t*)nullptr, (uint16_t*)nullptr,
+                       (const float*)nullptr, (float*)nullptr, 0.0f, (double*)nullptr);
     return hipGetLastError();
 }
 
-// n must be a multiple of 32 (the matrix-core tiles are 32 pixels wide).
+// Training forward: the same fused kernel, storing a1..a3 (centre tiles),
+// out, and with a target the loss gradient and the summed squared error.
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s) {
-    const dim3 gm(n / kMW, n / kMR, batch);
-    hipLaunchKernelGGL(k_conv_in_fwd, gm, dim3(256), 0, s, B.r0, B.w1b, prm, B.a1, n);
-    hipLaunchKernelGGL(k_conv_mid_fwd, gm, dim3(256), 0, s, B.a1, B.wb2, prm, kB2, B.a2, n);
-    hipLaunchKernelGGL(k_conv_mid_fwd, gm, dim3(256), 0, s, B.a2, B.wb3, prm, kB3, B.a3, n);
     const float inv = 1.0f / ((float)n * n * batch);
-    hipLaunchKernelGGL(k_conv_out_fwd, SKE_GRID(n, batch), dim3(256), 0, s, B.a3, B.r0, prm, B.out, target, dout, inv,
-                       loss, n);
+    hipLaunchKernelGGL(k_cnn_fused_fwd<true>, dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0, prm,
+                       B.w1b, B.wb2, B.wb3, B.out, n, B.a1, B.a2, B.a3, target, dout, inv, loss);
     return hipGetLastError();
 }
 

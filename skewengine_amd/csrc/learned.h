@@ -10,22 +10,22 @@ namespace ske {
 
 // Flat fp32 parameter vector: CNN weights and biases, then the symmetric
 // filter taps sym[0..ndet-1] (full filter h[m] = sym[|m|]).
-constexpr int kLC = 32;                 // hidden channels
-constexpr int kW1 = 0;                  // [32][9]       conv 1 -> 32
-constexpr int kB1 = kW1 + kLC * 9;      // [32]
-constexpr int kW2 = kB1 + kLC;          // [32][9][32]   conv 32 -> 32
+constexpr int kLC = 16;                 // hidden channels
+constexpr int kW1 = 0;                  // [16][9]       conv 1 -> 16
+constexpr int kB1 = kW1 + kLC * 9;      // [16]
+constexpr int kW2 = kB1 + kLC;          // [16][9][16]   conv 16 -> 16
 constexpr int kB2 = kW2 + kLC * 9 * kLC;
-constexpr int kW3 = kB2 + kLC;          // [32][9][32]   conv 32 -> 32
+constexpr int kW3 = kB2 + kLC;          // [16][9][16]   conv 16 -> 16
 constexpr int kB3 = kW3 + kLC * 9 * kLC;
-constexpr int kW4 = kB3 + kLC;          // [9][32]       conv 32 -> 1
+constexpr int kW4 = kB3 + kLC;          // [9][16]       conv 16 -> 1
 constexpr int kB4 = kW4 + 9 * kLC;      // [1]
 constexpr int kTaps = kB4 + 1;          // [ndet]
-static_assert(kTaps == 19105, "parameter layout");
+static_assert(kTaps == 4945, "parameter layout");
 
 // Per-pixel tensors of a batch (all batch x n x n pixels).
 struct LearnedBufs {
     const float* r0;     // FBP reconstruction, fp32
-    uint16_t* a1;        // bf16 NHWC, 32 channels
+    uint16_t* a1;        // bf16 NHWC, 16 channels
     uint16_t* a2;
     uint16_t* a3;
     float* out;          // fp32
@@ -39,7 +39,7 @@ struct LearnedBufs {
 };
 
 hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s);
-// Training forward: every activation stored (n a multiple of 32).
+// Training forward: a1..a3 stored (n a multiple of 16).
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s);
 // Inference forward, fused in LDS (no activations stored; n a multiple of 16).

@@ -126,7 +126,7 @@ class LearnedFBP:
     def download(self, which, batch=1):
         n = self.fbp.n
         if which in (A1, A2, A3):
-            out = np.empty((batch, n, n, 32), np.uint16)
+            out = np.empty((batch, n, n, _lib.LEARNED_CHANNELS), np.uint16)
         elif which == FILTERED_GRAD:
             out = np.empty((batch, self.fbp.n_angles, self.fbp.n_det), np.float32)
         else:

@@ -53,7 +53,7 @@ def test_forward_layers_match_fp64(n, nang, ndet, batch):
         assert np.array_equal(m.params(), prm)
         m.make_batch(batch, seed=7, i0=1e4, mu=1.0)
         sino = f.download(SINO, batch)
-        m.gradient(batch)  # the per-layer forward, which keeps the activations
+        m.gradient(batch)  # the training forward, which stores the activations
         r0 = m.download(lm.R0, batch)
         a = [m.download(w, batch) for w in (lm.A1, lm.A2, lm.A3)]
         out = f.download(RECON, batch)
