@@ -28,6 +28,9 @@ CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "learned.cpp", "png_io.cp
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-I" + INCLUDE, "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Per-source extras: the learned CNN keeps its matrix-core accumulators in
+# ordinary VGPRs (no accumulation-register copies after every MFMA).
+EXTRA = {"kernels_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def _run(cmd):
@@ -45,7 +48,7 @@ def build(verbose=False):
     objs = []
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
+        _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE + EXTRA.get(src, []))
         objs.append(obj)
     for src in CPP_SOURCES:
         obj = os.path.join(objdir, src.replace(".cpp", ".o"))

@@ -49,7 +49,8 @@ constexpr int T = 16;  // tile side
 // ---------------------------------------------------------------------------
 
 // W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout, and W1 as
-// bf16 [co][16] (taps 0..8, then zeros): the matrix-core operands.
+// bf16 [co][16] with k = 4*ky + kx (kx = 3 and k >= 12 zero): the
+// matrix-core operands.
 __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb2, uint16_t* __restrict__ wb3,
                                uint16_t* __restrict__ w1b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -58,8 +59,8 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
         wb3[i] = f2bf(prm[kW3 + i]);
     }
     if (i < 16 * C) {
-        const int co = i / 16, t = i % 16;
-        w1b[i] = t < 9 ? f2bf(prm[kW1 + co * 9 + t]) : (uint16_t)0;
+        const int co = i / 16, k = i % 16, ky = k / 4, kx = k % 4;
+        w1b[i] = (k < 12 && kx < 3) ? f2bf(prm[kW1 + co * 9 + 3 * ky + kx]) : (uint16_t)0;
     }
 }
 
@@ -77,51 +78,66 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // the initial accumulator).  Lane layout: A[i][k] at lane i + 16*(k/8),
 // element k%8; B[k][j] at lane j + 16*(k/8), element k%8; D[i][j] at lane
 // j + 16*(i/4), register i%4.
-//   conv 1 -> 16: k = tap (9 taps, then zeros): one MFMA per group;
+//   conv 1 -> 16: k = 4*ky + kx over kx = 0..3 (the kx = 3 weight is zero)
+//   for the three rows, then zeros: one MFMA per group; the operand is
+//   read as bf16 pairs (x, x+1) and (x+2, x+3) from two copies of the r0
+//   tile packed at even and at odd columns;
 //   conv 16 -> 16: k = (tap, channel) over the tap pairs (0,1), (2,3),
 //   (4,5), (6,7), (8,-): five MFMAs per group, taps in order.
 //   conv 16 -> 1: v_dot2c_f32_bf16 on channel pairs, channels 0-7 and 8-15
 //   in two partial sums (taps in order, pairs in order), then (s0+s1) + b4.
-// Activation tiles are bf16 NHWC in LDS with a 48-byte pixel pitch (16
-// channels + 8 pad), so the 16-byte operand reads of 8 consecutive lanes
-// fall in distinct banks.
+// Activation tiles are bf16 NHWC in LDS at a 32-byte pixel pitch (no pad:
+// a 48-byte pitch avoids bank conflicts but costs two workgroups per CU,
+// and measured slower).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 static_assert(C == 16, "the fused forward is written for 16 hidden channels");
-constexpr int kP = 24;           // LDS pixel pitch in bf16
+constexpr int kP = 16;           // LDS pixel pitch in bf16
 constexpr int kFT = 16;          // output tile side
 constexpr int kF0 = kFT + 8;     // r0 region side
 constexpr int kF1 = kFT + 6;     // a1
 constexpr int kF2 = kFT + 4;     // a2
 constexpr int kF3 = kFT + 2;     // a3
 constexpr int kFWaves = 4;
+constexpr int kS0 = 12;          // bf16 pairs per row of the packed r0 tile
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
 }
 
-// B operand of conv 1 -> 16 for the a1-region pixel (y, x): taps 8*kb ..
-// 8*kb+7 of the bf16 r0 tile (zeros past tap 8).
-__device__ __forceinline__ bf16x8 in_operand(const uint16_t* s0, int y, int x, int kb) {
-    uint32_t e[8];
-#pragma unroll
-    for (int t = 0; t < 8; t++) e[t] = 0;
+// B operand of conv 1 -> 16 for the a1-region pixel (y, x): k = 8*kb ..
+// 8*kb+7 with k = 4*ky + kx, i.e. the pairs (x, x+1), (x+2, x+3) of tile
+// rows y, y+1 (kb = 0) or y+2 (kb = 1); `se`/`so` hold the pairs starting at
+// even / odd columns.
+__device__ __forceinline__ bf16x8 in_operand(const uint32_t* se, const uint32_t* so, int y, int x, int kb) {
+    const uint32_t* t = ((x & 1) ? so : se) + (x >> 1);
+    uint4 v = make_uint4(0, 0, 0, 0);
     if (kb == 0) {
-#pragma unroll
-        for (int t = 0; t < 8; t++) e[t] = s0[(y + t / 3) * kF0 + x + t % 3];
+        v.x = t[y * kS0];
+        v.y = t[y * kS0 + 1];
+        v.z = t[(y + 1) * kS0];
+        v.w = t[(y + 1) * kS0 + 1];
     } else if (kb == 1) {
-        e[0] = s0[(y + 2) * kF0 + x + 2];
+        v.x = t[(y + 2) * kS0];
+        v.y = t[(y + 2) * kS0 + 1];
     }
-    return as_bf16x8(make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16)));
+    return as_bf16x8(v);
 }
 
 // relu, bf16, and a lane's 4 channels of one pixel (8 bytes); `keep` false
-// writes zeros.
+// writes zeros.  The relu acts on the rounded pairs as int16 maxima with 0
+// (a bf16 is negative exactly when its int16 pattern is), which equals
+// rounding the relu of the fp32 values.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    const s16x2 m = __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0});
+    return __builtin_bit_cast(uint32_t, m);
+}
 __device__ __forceinline__ void store4(uint16_t* dst, const f32x4& acc, bool keep) {
     uint2 v;
-    v.x = pack_bf16(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f));
-    v.y = pack_bf16(fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+    v.x = relu_bf16x2(pack_bf16(acc[0], acc[1]));
+    v.y = relu_bf16x2(pack_bf16(acc[2], acc[3]));
     if (!keep) v = make_uint2(0, 0);
     *(uint2*)dst = v;
 }
@@ -142,6 +158,7 @@ __device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[5], const uint16_t*
 // wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
 // groups g and g + kFWaves together (two independent accumulators).  Lanes
 // of the absent tap 9 read tap 8 again; their weights are zero.
+template <bool kEdge>
 __device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
                                           const bf16x8 (&w)[5], const f32x4 bias, int wave, int j16, int kb) {
     const int wo = wi - 2, count = wo * wo;
@@ -169,7 +186,7 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], v1, acc1, 0, 0, 0);
             }
             const int iy = oy + y1, ix = ox + x1;
-            if (p1 < count) store4(out + p1 * kP + 4 * kb, acc1, iy >= 0 && iy < n && ix >= 0 && ix < n);
+            if (p1 < count) store4(out + p1 * kP + 4 * kb, acc1, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
         } else {
 #pragma unroll
             for (int tp = 0; tp < 5; tp++)
@@ -177,7 +194,7 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int
                                                                0, 0, 0);
         }
         const int iy = oy + y0, ix = ox + x0;
-        if (p0 < count) store4(out + p0 * kP + 4 * kb, acc0, iy >= 0 && iy < n && ix >= 0 && ix < n);
+        if (p0 < count) store4(out + p0 * kP + 4 * kb, acc0, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
     }
 }
 
@@ -193,23 +210,17 @@ __device__ __forceinline__ void store_centre(const uint16_t* reg, int wi, int of
     }
 }
 
-template <bool kTrain>
-__global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __restrict__ r0,
-                                                                const float* __restrict__ prm,
-                                                                const uint16_t* __restrict__ w1b,
-                                                                const uint16_t* __restrict__ wb2,
-                                                                const uint16_t* __restrict__ wb3,
-                                                                float* __restrict__ out, int n,
-                                                                uint16_t* __restrict__ a1g, uint16_t* __restrict__ a2g,
-                                                                uint16_t* __restrict__ a3g,
-                                                                const float* __restrict__ target,
-                                                                float* __restrict__ dout, float inv_count,
-                                                                double* __restrict__ loss) {
-    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kP];  // a1, then a3
-    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kP];  // a2
-    __shared__ uint16_t s0[kF0 * kF0];                    // r0 as bf16
-    __shared__ uint32_t w4[9 * C / 2];                    // W4 as bf16 pairs
-    __shared__ float red[64 * kFWaves];
+// One output tile.  kEdge: some region pixel lies outside the image (zero
+// padding to apply); interior tiles skip the bounds tests.
+template <bool kTrain, bool kEdge>
+__device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const float* __restrict__ prm,
+                                         const uint16_t* __restrict__ w1b, const uint16_t* __restrict__ wb2,
+                                         const uint16_t* __restrict__ wb3, float* __restrict__ out, int n,
+                                         uint16_t* __restrict__ a1g, uint16_t* __restrict__ a2g,
+                                         uint16_t* __restrict__ a3g, const float* __restrict__ target,
+                                         float* __restrict__ dout, float inv_count, double* __restrict__ loss,
+                                         uint16_t* sa, uint16_t* sb, uint32_t* s0e, uint32_t* s0o, uint32_t* w4,
+                                         float* red) {
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
@@ -226,31 +237,58 @@ __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __r
         bias2[r] = prm[kB2 + 4 * kb + r];
         bias3[r] = prm[kB3 + 4 * kb + r];
     }
-    for (int t = threadIdx.x; t < kF0 * kF0; t += 64 * kFWaves) {
-        const int ii = i0 - 4 + t / kF0, jj = j0 - 4 + t % kF0;
-        s0[t] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? f2bf(img[(size_t)ii * n + jj]) : (uint16_t)0;
+    {   // all of this thread's r0 loads in flight together
+        constexpr int kIt = (kF0 * kS0 + 64 * kFWaves - 1) / (64 * kFWaves);
+        float v[kIt][3];
+#pragma unroll
+        for (int it = 0; it < kIt; it++) {
+            const int t = threadIdx.x + 64 * kFWaves * it;
+            const int row = t / kS0, c = 2 * (t % kS0);  // columns c, c+1, c+2 of the tile (c+2 may be the pad 24)
+            const int ii = i0 - 4 + row;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int jj = j0 - 4 + c + q;
+                v[it][q] = (t < kF0 * kS0 && c + q < kF0 && (!kEdge || (ii >= 0 && ii < n && jj >= 0 && jj < n)))
+                               ? img[(size_t)ii * n + jj]
+                               : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kIt; it++) {
+            const int t = threadIdx.x + 64 * kFWaves * it;
+            if (t < kF0 * kS0) {
+                s0e[t] = pack_bf16(v[it][0], v[it][1]);
+                s0o[t] = pack_bf16(v[it][1], v[it][2]);
+            }
+        }
     }
     for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
     __syncthreads();
     {   // a1 on the 22 x 22 region at (i0-3, j0-3)
         const int count = kF1 * kF1;
-        for (int g = wave; g * 16 < count; g += kFWaves) {
-            const int p = g * 16 + j16;
-            const int pc = min(p, count - 1);
-            const int y = pc / kF1, x = pc % kF1;
-            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, in_operand(s0, y, x, kb), bias1, 0, 0, 0);
-            const int iy = i0 - 3 + y, ix = j0 - 3 + x;
-            if (p < count) store4(sa + p * kP + 4 * kb, acc, iy >= 0 && iy < n && ix >= 0 && ix < n);
+        for (int g0 = wave; g0 * 16 < count; g0 += 2 * kFWaves) {  // two groups at a time
+            const int g1 = g0 + kFWaves;
+            const int p0 = g0 * 16 + j16, p1 = g1 * 16 + j16;
+            const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
+            const int y0 = q0 / kF1, x0 = q0 % kF1, y1 = q1 / kF1, x1 = q1 % kF1;
+            const bf16x8 v0 = in_operand(s0e, s0o, y0, x0, kb), v1 = in_operand(s0e, s0o, y1, x1, kb);
+            const f32x4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, v0, bias1, 0, 0, 0);
+            const f32x4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, v1, bias1, 0, 0, 0);
+            int iy = i0 - 3 + y0, ix = j0 - 3 + x0;
+            if (p0 < count) store4(sa + p0 * kP + 4 * kb, acc0, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
+            iy = i0 - 3 + y1;
+            ix = j0 - 3 + x1;
+            if (p1 < count) store4(sa + p1 * kP + 4 * kb, acc1, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
         }
     }
     __syncthreads();
     const size_t img_off = (size_t)b * n * n * C;
     if (kTrain) store_centre(sa, kF1, 3, a1g + img_off, i0, j0, n);
-    mid_layer(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
+    mid_layer<kEdge>(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
     load_mid_weights(w, wb3, j16, kb);  // in flight while the other waves finish
     __syncthreads();
     if (kTrain) store_centre(sb, kF2, 2, a2g + img_off, i0, j0, n);
-    mid_layer(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
+    mid_layer<kEdge>(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
     if (kTrain) store_centre(sa, kF3, 1, a3g + img_off, i0, j0, n);
     // conv 16 -> 1 + bias + residual: one thread per pixel and channel half
@@ -290,6 +328,34 @@ __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __r
         }
         if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
     }
+}
+
+
+template <bool kTrain>
+__global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __restrict__ r0,
+                                                                const float* __restrict__ prm,
+                                                                const uint16_t* __restrict__ w1b,
+                                                                const uint16_t* __restrict__ wb2,
+                                                                const uint16_t* __restrict__ wb3,
+                                                                float* __restrict__ out, int n,
+                                                                uint16_t* __restrict__ a1g, uint16_t* __restrict__ a2g,
+                                                                uint16_t* __restrict__ a3g,
+                                                                const float* __restrict__ target,
+                                                                float* __restrict__ dout, float inv_count,
+                                                                double* __restrict__ loss) {
+    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kP];  // a1, then a3
+    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kP];  // a2
+    __shared__ uint32_t s0e[kF0 * kS0], s0o[kF0 * kS0];   // r0 as bf16 pairs at even / odd columns
+    __shared__ uint32_t w4[9 * C / 2];                    // W4 as bf16 pairs
+    __shared__ float red[64 * kFWaves];
+    const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
+    const bool interior = i0 >= 4 && i0 + kFT + 4 <= n && j0 >= 4 && j0 + kFT + 4 <= n;  // uniform
+    if (interior)
+        cnn_tile<kTrain, false>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa, sb,
+                                s0e, s0o, w4, red);
+    else
+        cnn_tile<kTrain, true>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa, sb,
+                               s0e, s0o, w4, red);
 }
 
 // ---------------------------------------------------------------------------

@@ -22,6 +22,7 @@ struct ske_learned {
     ske_fbp* f = nullptr;
     int nparam = 0;
     int step = 0;                  // Adam steps taken
+    bool derived = false;          // full taps and bf16 weights match prm (cleared whenever prm changes)
     DevBuf<float> prm, grad, m1, m2, full, dq, dfull, dout;
     DevBuf<double> loss;
     DevBuf<float> r0, dz, dr0;     // dz: three layers back to back
@@ -94,9 +95,15 @@ int forward(ske_learned* l, int batch, bool train) {
     hipStream_t s = f->stream;
     const FbpGeom& g = f->g;
     const LearnedBufs B = l->bufs(batch, train);
+    if (!l->derived) {  // the parameters changed: expanded taps and bf16 matrix-core weights
+        SKE_TRY(f->timer.launch(s, "learned_params", [&] {
+            hipError_t e = launch_taps_expand(l->prm.p + kTaps, l->full.p, g.ndet, s);
+            if (e != hipSuccess) return e;
+            return launch_pack_weights(l->prm.p, l->wb.p, l->wb.p + 9 * kLC * kLC, l->wb.p + 2 * 9 * kLC * kLC, s);
+        }));
+        l->derived = true;
+    }
     SKE_TRY(f->timer.launch(s, "learned_filter", [&] {
-        hipError_t e = launch_taps_expand(l->prm.p + kTaps, l->full.p, g.ndet, s);
-        if (e != hipSuccess) return e;
         return launch_fbp_filter(f->buf[SKE_FBP_SINO].p, f->buf[SKE_FBP_FILTERED].p, l->full.p, g, batch, s);
     }));
     SKE_TRY(f->timer.launch(s, "learned_backproject", [&] {
@@ -104,13 +111,9 @@ int forward(ske_learned* l, int batch, bool train) {
                                       s);
     }));
     return f->timer.launch(s, "cnn_forward", [&] {
-        hipError_t e = launch_pack_weights(l->prm.p, l->wb.p, l->wb.p + 9 * kLC * kLC, l->wb.p + 2 * 9 * kLC * kLC, s);
-        if (e != hipSuccess) return e;
         if (!train) return launch_cnn_infer(B, l->prm.p, g.n, batch, s);
-        if (train) {
-            e = launch_fill_f64(l->loss.p, 1, s);
-            if (e != hipSuccess) return e;
-        }
+        hipError_t e = launch_fill_f64(l->loss.p, 1, s);
+        if (e != hipSuccess) return e;
         return launch_cnn_forward(B, l->prm.p, g.n, batch, train ? f->buf[SKE_FBP_IMAGE].p : nullptr,
                                   train ? l->dout.p : nullptr, train ? l->loss.p : nullptr, s);
     });
@@ -216,6 +219,7 @@ int ske_learned_init(ske_learned* l, unsigned long long seed) {
 int ske_learned_set_params(ske_learned* l, const float* params) {
     if (!l || !params) return fail(SKE_EINVAL, "learned or params is NULL");
     SKE_TRY(l->prm.upload(params, l->nparam, l->f->stream));
+    l->derived = false;
     l->step = 0;
     if (l->m1.p) {
         SKE_HIP(launch_fill_f32(l->m1.p, l->nparam, 0.0f, l->f->stream));
@@ -259,6 +263,7 @@ int ske_learned_train_step(ske_learned* l, int batch, float lr, float lr_taps, d
         return launch_adam(l->prm.p, l->grad.p, l->m1.p, l->m2.p, l->nparam, lr, lr_taps, kTaps, l->step,
                            l->f->stream);
     }));
+    l->derived = false;
     return read_loss(l, batch, loss);
 }
 
