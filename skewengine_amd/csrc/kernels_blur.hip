@@ -20,14 +20,38 @@
 
 namespace ske {
 
-// tab layout (double): [0,128): cos/sin(2 pi t/64) interleaved; [128,160): cos(2 pi t/32).
+// Natural log of a double: x = m 2^e with m in [1, 2); the top 8 fraction
+// bits i of m pick c_i ~ 1/(1 + (i + 1/2)/256) from the table, so that
+// u = m c_i - 1 (one fma) has |u| < 2^-9, and
+//     log x = e ln2 + (-log c_i) + log1p(u),
+//     log1p(u) = u - u^2/2 + u^3/3 - u^4/4 + u^5/5   (the rest < 2^-55).
+// -log c_i and ln 2 (as a double-double) come from the host in extended
+// precision.  x = 0 gives -inf like the libm log.  `lt` = the table in LDS.
+__device__ __forceinline__ double fast_log(double x, const double* lt) {
+    const double mant = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    const int ex = __builtin_amdgcn_frexp_exp(x) - 1;
+    const int i = (int)((uint32_t)(__double_as_longlong(mant) >> 44) & 0xFFu);
+    const double c = lt[2 * i], t = lt[2 * i + 1];
+    const double u = __builtin_fma(2.0 * mant, c, -1.0);
+    const double q = __builtin_fma(u, __builtin_fma(u, __builtin_fma(u, 0.2, -0.25), 1.0 / 3.0), -0.5);
+    const double p = __builtin_fma(u * u, q, u);
+    const double e = (double)ex;
+    const double r = __builtin_fma(e, lt[513], t + p);
+    const double y = __builtin_fma(e, lt[512], r);
+    return x == 0.0 ? -INFINITY : y;
+}
+
+// tab layout (double): [0,128): cos/sin(2 pi t/64) interleaved; [128,160): cos(2 pi t/32);
+// from kLogTab the log table of fast_log.
 __global__ __launch_bounds__(64) void k_op_blur(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
                                                 const float* __restrict__ in4, const int* __restrict__ target,
                                                 const float* __restrict__ weight, const uint32_t* __restrict__ fl,
                                                 const float* __restrict__ Sbuf, const double* __restrict__ tab,
-                                                float* __restrict__ arena) {
+                                                float* __restrict__ arena, int p_maxL) {
     extern __shared__ float lds[];
     float* row = lds + kPad;
+    double* lt = reinterpret_cast<double*>(lds) + ((size_t)(p_maxL + 2 * kPad) + 1) / 2;  // after the row
+    for (int t = threadIdx.x; t < 514; t += 64) lt[t] = tab[kLogTab + t];
     const RowWork w = work[blockIdx.x];
     const SecDesc sd = secs[w.sec];
     const int r = w.row;
@@ -71,7 +95,7 @@ __global__ __launch_bounds__(64) void k_op_blur(const SecDesc* __restrict__ secs
         }
         double Lg[32];
 #pragma unroll
-        for (int k = 0; k < 32; ++k) Lg[k] = log(Xr[k] * Xr[k] + Xi[k] * Xi[k]);
+        for (int k = 0; k < 32; ++k) Lg[k] = fast_log(Xr[k] * Xr[k] + Xi[k] * Xi[k], lt);
         double vv[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -103,9 +127,10 @@ __global__ __launch_bounds__(64) void k_op_blur(const SecDesc* __restrict__ secs
 hipError_t launch_op_blur(const DevPlan& p, const float* in4, const float* Sbuf, float* arena, const double* tab,
                           hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
-    const size_t lds = sizeof(float) * (size_t)(p.maxL + 2 * kPad);
+    const size_t row_floats = ((size_t)(p.maxL + 2 * kPad) + 1) / 2 * 2;  // keep the log table 8-byte aligned
+    const size_t lds = sizeof(float) * row_floats + sizeof(double) * 514;
     hipLaunchKernelGGL(k_op_blur, dim3(p.nwork), dim3(64), lds, s, p.secs, p.work, in4, p.target, p.weight, p.fl,
-                       Sbuf, tab, arena);
+                       Sbuf, tab, arena, p.maxL);
     return hipGetLastError();
 }
 

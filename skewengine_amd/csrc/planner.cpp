@@ -198,7 +198,7 @@ void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off) {
 }
 
 void cepstrum_table(std::vector<double>& tab) {
-    tab.assign(192, 0.0);
+    tab.assign(kLogTab + 2 * 256 + 2, 0.0);
     for (int t = 0; t < 64; t++) {
         tab[2 * t] = cos(2.0 * M_PI * t / 64.0);
         tab[2 * t + 1] = sin(2.0 * M_PI * t / 64.0);
@@ -207,6 +207,16 @@ void cepstrum_table(std::vector<double>& tab) {
     // src/kernelRADN.h:152-162: cutoff 0.4 of the 64-sample window
     const double cutoff = 0.4, dws = 64;
     for (int k = 0; k < 26; k++) tab[160 + k] = sin((k / (cutoff * dws - 1)) * (M_PI / 2));
+    // natural-log table: for the top 8 mantissa bits i, c_i = RN(1 / (1 + (i + 1/2)/256)) and
+    // T_i = -log(c_i) (long double, then rounded), then ln 2 as a double-double
+    for (int i = 0; i < 256; i++) {
+        const double c = (double)(1.0L / (1.0L + (i + 0.5L) / 256.0L));
+        tab[kLogTab + 2 * i] = c;
+        tab[kLogTab + 2 * i + 1] = (double)(-logl((long double)c));
+    }
+    const long double ln2 = logl(2.0L);
+    tab[kLogTab + 512] = (double)ln2;
+    tab[kLogTab + 513] = (double)(ln2 - (long double)(double)ln2);
 }
 
 void lrt_gauss_table(std::vector<double>& tab) {

@@ -69,8 +69,11 @@ void recip_table(int n, std::vector<float>& tab4);
 // Within one N, stage n (n = N, N/2, ..., 2) occupies pairs [N-n, N-n+n/2).
 void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off);
 
-// cos/sin(2 pi t/64) interleaved (t < 64), then cos(2 pi t/32) (t < 32),
-// then the local Radon transform's sine ramp sin(k/24.6 * pi/2), k <= 25.
+// Cepstrum/local-Radon constants (doubles): [0,128) cos/sin(2 pi t/64)
+// interleaved, [128,160) cos(2 pi t/32), [160,186) the local Radon's sine
+// ramp sin(k/24.6 * pi/2), then from kLogTab (ske_types.h) the natural-log
+// table of the cepstrum kernel (256 pairs c_i, -log c_i, then ln 2 as a
+// double-double).
 void cepstrum_table(std::vector<double>& tab);
 
 // Gaussian window of the local Radon transform, 64 weights per folded angle

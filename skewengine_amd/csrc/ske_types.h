@@ -10,6 +10,7 @@ namespace ske {
 // row plus two 64-float guard bands in LDS.
 constexpr int kMaxDim = 16000;
 constexpr int kPad = 64;        // LDS guard band on each side of a staged row
+constexpr int kLogTab = 192;    // offset of the log table in the cepstrum constants (planner.cpp)
 constexpr int kNumSectors = 180; // maxSector, src/global.cpp:69
 
 // One sector of one image, as the device sees it.  Mirrors the public fields
