@@ -131,7 +131,8 @@ hipError_t launch_op_unit(const DevPlan& p, const float* in4, const float* Sbuf,
 
 // ---------------------------------------------------------------------------
 // deskew (src/skewEngine.cuh:755-793) of all sectors of a plan, one thread per
-// oriented output cell, sectors in ascending order per orientation.
+// column of a few oriented output rows, sectors in ascending order per
+// orientation.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float stdmaxf(float a, float b) { return (a < b) ? b : a; }
 
@@ -159,7 +160,51 @@ __device__ __forceinline__ float so_cell(const SecDesc* __restrict__ secs, int s
     return 0.0f;
 }
 
+// Tap (r, j) of sector si with the common case r < H read without branches:
+// the row index is clamped into the sector (in bounds of its H x dim_i slot)
+// and the value masked by the row's first/last range afterwards.  Rows r >= H
+// are patched by the caller through so_cell.
+template <int kSource>
+__device__ __forceinline__ float so_tap(const SecDesc& sd, const uint32_t* __restrict__ fl,
+                                        const float* __restrict__ src, int r, int j) {
+    if (kSource == kDeskewDense) return src[(size_t)r * sd.dim_i + j];
+    const int rc = min(r, sd.H - 1);
+    const uint32_t v = fl[sd.rows + rc];
+    const float x = (kSource == kDeskewArena) ? src[sd.so + (size_t)rc * sd.dim_i + j] : src[sd.q + rc];
+    return (j >= fl_first(v) && j < fl_last(v)) ? x : 0.0f;
+}
+
+template <int kMax>
+__device__ __forceinline__ float deskew_acc(float O, float a, float b, float wj, int angle) {
+    if (kMax) {
+        O = stdmaxf(O, stdmaxf(a, b));
+        return (float)(angle + 1000 * (int)(O / 1000));
+    }
+    return O + ((1.0f - wj) * a + wj * b);
+}
+
+// One output cell over sectors [sb, se), in ascending order.
 template <int kSource, int kMax>
+__device__ void deskew_cell(const SecDesc* __restrict__ secs, const int* __restrict__ target,
+                            const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                            const float* __restrict__ src, int sb, int se, int i, int j, float* o) {
+    float O = *o;
+    for (int si = sb; si < se; ++si) {
+        const SecDesc& sd = secs[si];
+        const int tj = target[sd.tab + j];
+        const float wj = weight[sd.tab + j];
+        const float a = so_cell<kSource>(secs, si, sd, fl, src, i + tj, j);
+        const float b = so_cell<kSource>(secs, si, sd, fl, src, i + tj + 1, j);
+        O = deskew_acc<kMax>(O, a, b, wj, sd.angle);
+    }
+    *o = O;
+}
+
+// kRows consecutive output rows per thread: the kRows + 1 taps of a sector
+// are shared by neighbouring rows.  kU sectors per pass: their table reads,
+// then their gathers are issued together, and each row accumulates in
+// ascending sector order afterwards.
+template <int kSource, int kMax, int kRows, int kU>
 __global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs, const int* __restrict__ target,
                                                 const float* __restrict__ weight, const uint32_t* __restrict__ fl,
                                                 const float* __restrict__ src, int4 rlo, int4 rhi,
@@ -170,38 +215,81 @@ __global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs
     if (sb >= se) return;
     const bool tr = (t == 1 || t == 2);
     const int dim_i = tr ? dimy : dimx, dim_o = tr ? dimx : dimy;
-    const int i = blockIdx.y;
+    const int i0 = blockIdx.y * kRows;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= dim_o || j >= dim_i) return;
+    if (i0 >= dim_o || j >= dim_i) return;
     const size_t N = (size_t)dimx * dimy;
-    float* o = acc + (size_t)t * N + (size_t)i * dim_i + j;
-    float O = *o;
-    for (int si = sb; si < se; ++si) {
-        const SecDesc sd = secs[si];
-        const int tj = target[sd.tab + j];
-        const float wj = weight[sd.tab + j];
-        const float a = so_cell<kSource>(secs, si, sd, fl, src, i + tj, j);
-        const float b = so_cell<kSource>(secs, si, sd, fl, src, i + tj + 1, j);
-        if (kMax) {
-            O = stdmaxf(O, stdmaxf(a, b));
-            O = (float)(sd.angle + 1000 * (int)(O / 1000));
-        } else {
-            O += (1.0f - wj) * a + wj * b;
+    float* o = acc + (size_t)t * N + (size_t)i0 * dim_i + j;
+    if (i0 + kRows > dim_o) {  // last rows of the image
+        for (int k = 0; k < dim_o - i0; ++k)
+            deskew_cell<kSource, kMax>(secs, target, weight, fl, src, sb, se, i0 + k, j, o + (size_t)k * dim_i);
+        return;
+    }
+    float O[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) O[k] = o[(size_t)k * dim_i];
+    int si = sb;
+    for (; si + kU <= se; si += kU) {
+        int tj[kU];
+        float wj[kU], v[kU][kRows + 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const SecDesc& sd = secs[si + u];
+            tj[u] = target[sd.tab + j];
+            wj[u] = weight[sd.tab + j];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+#pragma unroll
+            for (int m = 0; m <= kRows; ++m) v[u][m] = so_tap<kSource>(secs[si + u], fl, src, i0 + tj[u] + m, j);
+        }
+        if (kSource != kDeskewDense) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const SecDesc& sd = secs[si + u];
+                if (i0 + tj[u] + kRows >= sd.H) {
+#pragma unroll
+                    for (int m = 0; m <= kRows; ++m)
+                        if (i0 + tj[u] + m >= sd.H) v[u][m] = so_cell<kSource>(secs, si + u, sd, fl, src, i0 + tj[u] + m, j);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int angle = secs[si + u].angle;
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) O[k] = deskew_acc<kMax>(O[k], v[u][k], v[u][k + 1], wj[u], angle);
         }
     }
-    *o = O;
+    for (; si < se; ++si) {
+        const SecDesc& sd = secs[si];
+        const int tj = target[sd.tab + j];
+        const float wj = weight[sd.tab + j];
+        float v[kRows + 1];
+#pragma unroll
+        for (int m = 0; m <= kRows; ++m) v[m] = so_cell<kSource>(secs, si, sd, fl, src, i0 + tj + m, j);
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) O[k] = deskew_acc<kMax>(O[k], v[k], v[k + 1], wj, sd.angle);
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) o[(size_t)k * dim_i] = O[k];
 }
+
+// 16 rows, one sector per pass: measured fastest on the batched viewshed
+// (129 VGPRs; 8 rows 1.2x, 4 rows 1.9x, 1 row 4.1x slower; a second sector per
+// pass only adds registers).
+constexpr int kDeskewRows = 16, kDeskewU = 1;
 
 hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source, int is_max,
                          float* acc, int dimx, int dimy, long long img_acc, hipStream_t s) {
     const int dl = dimx > dimy ? dimx : dimy;
-    dim3 grid((dl + 255) / 256, dl, 4);
+    dim3 grid((dl + 255) / 256, (dl + kDeskewRows - 1) / kDeskewRows, 4);
     int4 lo = make_int4(range[0], range[1], range[2], range[3]);
     int4 hi = make_int4(range[4], range[5], range[6], range[7]);
     float* a = acc + img_acc;
-#define SKE_DESKEW(SRC, MX)                                                                              \
-    hipLaunchKernelGGL((k_deskew<SRC, MX>), grid, dim3(256), 0, s, p.secs, p.target, p.weight, p.fl, src, \
-                       lo, hi, a, dimx, dimy)
+#define SKE_DESKEW(SRC, MX)                                                                                  \
+    hipLaunchKernelGGL((k_deskew<SRC, MX, kDeskewRows, kDeskewU>), grid, dim3(256), 0, s, p.secs, p.target, \
+                       p.weight, p.fl, src, lo, hi, a, dimx, dimy)
     if (source == kDeskewDense) {
         if (is_max) SKE_DESKEW(kDeskewDense, 1); else SKE_DESKEW(kDeskewDense, 0);
     } else if (source == kDeskewArena) {
