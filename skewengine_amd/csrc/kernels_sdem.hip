@@ -138,11 +138,8 @@ constexpr int kTabLo = 16, kTabHi = 32;
 constexpr int kTabN = kTabLo + kChunksMax + kTabHi;
 enum { kTabL0 = 0, kTabL1 = 1, kTabL2 = 2, kTabL3 = 3, kTabSuf = 4, kTabPre = 5, kTabs = 6 };
 
-// LDS of the bit-block kernels: guard band, row, guard band, chunk tables,
-// and for lane compaction (group > 1) a list of 64 x group ints per wave.
-inline size_t sdem_bits_lds(int maxL, int waves = 4, int group = 1) {
-    return sizeof(float) * ((size_t)maxL + 2 * kPadBits + kTabs * kTabN + (group > 1 ? 64 * waves * group : 0));
-}
+// LDS of the bit-block kernels: guard band, row, guard band, chunk tables.
+inline size_t sdem_bits_lds(int maxL) { return sizeof(float) * ((size_t)maxL + 2 * kPadBits + kTabs * kTabN); }
 
 __device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) {
     const int pc = __popc(w);
@@ -306,10 +303,11 @@ __device__ __forceinline__ void row_tables(const float* row, int L, float* tab) 
 // After each 32-step block, lanes whose scan is over retire (write their cv)
 // and, once the live scans fit in fewer slots, they are packed into the
 // lowest slots in (slot, lane) order: the p-th live scan moves to slot p/64,
-// lane p%64.  Per-lane state moves with ds_bpermute; the source of each
-// destination comes from a small per-wave LDS index list.  The p-th live
-// scan of a slot k satisfies p < 64(k+1), so destination slot k2 draws only
-// from slots >= k2 and the packing can run in place, lowest slot first.
+// lane p%64.  Per-lane state moves with ds_bpermute; each destination lane
+// finds its source from the slots' live masks (prefix counts, then the n-th
+// set bit), so the packing needs no LDS.  Slots 0..k-1 hold at most 64k live
+// scans, so destination slot k2 draws only from slots >= k2 and the packing
+// can run in place, lowest slot first.
 // A lane's steps past its row end are recorded as visible by or-ing the low
 // bits of its block word; retired and empty lanes read from a position whose
 // remaining steps stay inside the staged row and guard bands.  The
@@ -317,10 +315,32 @@ __device__ __forceinline__ void row_tables(const float* row, int L, float* tab) 
 // pass of the same group (same wave, later) adds its cv and stores the float
 // (float)(cvR + cvL) * R, the same value as the one-chunk path.
 // ---------------------------------------------------------------------------
+// Position of the r-th (0-based) set bit of m; m must have more than r set bits.
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int pos = 0;
+    uint32_t w = (uint32_t)m;
+    const int c0 = __popc(w);
+    if (r >= c0) {
+        r -= c0;
+        pos = 32;
+        w = (uint32_t)(m >> 32);
+    }
+    int c = __popc(w & 0xFFFFu);
+    if (r >= c) { r -= c; pos += 16; w >>= 16; }
+    c = __popc(w & 0xFFu);
+    if (r >= c) { r -= c; pos += 8; w >>= 8; }
+    c = __popc(w & 0xFu);
+    if (r >= c) { r -= c; pos += 4; w >>= 4; }
+    c = __popc(w & 0x3u);
+    if (r >= c) { r -= c; pos += 2; w >>= 2; }
+    if (r >= (int)(w & 1u)) pos += 1;
+    return pos;
+}
+
 template <int kDir, int kG>
 __device__ __forceinline__ void scan_group(const float* row, int L, float povh, int c0, int nck,
                                            const float* __restrict__ tab, const float4* __restrict__ rtab,
-                                           float* __restrict__ so, float R, int* __restrict__ idx, int& nblk) {
+                                           float* __restrict__ so, float R, int& nblk) {
     const float2* __restrict__ ytab = reinterpret_cast<const float2*>(rtab + kMaxDim + 1);
     int* __restrict__ soi = reinterpret_cast<int*>(so);
     const float invL = ytab[L].x;
@@ -388,25 +408,35 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
         T = __builtin_amdgcn_readfirstlane(T);
         const int nn = (T + 63) >> 6;
         if (nn < ns) {
-            int base = 0;
+            // live scans before each slot (wave-uniform), from the masks as
+            // they were before any slot is rewritten
+            uint64_t am[kG];
+            int pre[kG];
+            int acc = 0;
 #pragma unroll
             for (int k = 0; k < kG; ++k) {
-                if (k < ns) {
-                    const uint64_t m = act[k];
-                    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    if ((m >> lane) & 1) idx[base + below] = 64 * k + lane;
-                    base += (int)__popcll(m);
-                }
+                am[k] = k < ns ? act[k] : 0ull;
+                pre[k] = acc;
+                acc += (int)__popcll(am[k]);
             }
-            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k2 = 0; k2 < kG; ++k2) {
                 if (k2 < nn) {
+                    // the p-th live scan lies in the last slot k with pre[k] <= p
+                    // (k >= k2, since p >= 64 k2 >= pre[k2]) at its (p - pre[k])-th live lane
                     const int p = 64 * k2 + lane;
-                    const int src = p < T ? idx[p] : -1;
-                    const int sk = src >> 6;          // -1 for lanes left empty
-                    const int sa = (src & 63) << 2;   // ds_bpermute byte address
+                    uint64_t m = am[k2];
+                    int base = pre[k2], sk = k2;
+#pragma unroll
+                    for (int k = k2 + 1; k < kG; ++k) {
+                        if (k < ns && p >= pre[k]) {
+                            m = am[k];
+                            base = pre[k];
+                            sk = k;
+                        }
+                    }
+                    if (p >= T) sk = -1;  // lane left empty
+                    const int sa = nth_set_bit(m, p >= T ? 0 : p - base) << 2;  // ds_bpermute byte address
                     int nj = safe, nA = 1, nS = 0;
                     float nh = 0.0f, nM = -2000.0f;
 #pragma unroll
@@ -434,7 +464,6 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
                     act[k2] = __builtin_amdgcn_ballot_w64(p < T);
                 }
             }
-            __builtin_amdgcn_wave_barrier();
             ns = nn;
         }
     }
@@ -469,13 +498,12 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
     float* so = arena + sd.so + (size_t)r * sd.dim_i + f;
     int nblk = 0;  // 32-step blocks this wave ran (both directions)
     if (kG > 1) {
-        int* idx = reinterpret_cast<int*>(tab + kTabs * kTabN) + wave * 64 * kG;
         const int ngroups = (nchunk + kG - 1) / kG;
         for (int g = wave; g < ngroups;) {
             const int c0 = g * kG;
             const int nck = min(kG, nchunk - c0);
-            scan_group<+1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, idx, nblk);
-            scan_group<-1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, idx, nblk);
+            scan_group<+1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, nblk);
+            scan_group<-1, kG>(row, L, povh, c0, nck, tab, rtab, so, sd.R, nblk);
             int ng = 0;
             if (lane == 0) ng = atomicAdd(gnext, 1);
             g = __builtin_amdgcn_readfirstlane(ng);
@@ -554,7 +582,7 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
     if (p.nwork == 0) return hipSuccess;
     if (p.maxL > 4096) return launch_op_sdem(p, in4, Sbuf, arena, rtab, povh, 2, s);
     const int G = t.group > 1 ? 4 : 1;
-    const size_t lds = sdem_bits_lds(p.maxL, t.waves, G);
+    const size_t lds = sdem_bits_lds(p.maxL);
     if (t.persistent_wgs_per_cu > 0 && counter) {
         int dev = 0, ncu = 0;
         hipError_t e = hipGetDevice(&dev);
