@@ -141,12 +141,28 @@ enum { kTabL0 = 0, kTabL1 = 1, kTabL2 = 2, kTabL3 = 3, kTabSuf = 4, kTabPre = 5,
 // LDS of the bit-block kernels: guard band, row, guard band, chunk tables.
 inline size_t sdem_bits_lds(int maxL) { return sizeof(float) * ((size_t)maxL + 2 * kPadBits + kTabs * kTabN); }
 
+__device__ __forceinline__ int lshl1_add(int a, int b) {
+    int r;
+    asm("v_lshl_add_u32 %0, %1, 1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ void flush_block(uint32_t w, int d0, int& A, int& s) {
     const int pc = __popc(w);
-    const int sp = __popc(w & 0xAAAAAAAAu) + 2 * __popc(w & 0xCCCCCCCCu) + 4 * __popc(w & 0xF0F0F0F0u) +
-                   8 * __popc(w & 0xFF00FF00u) + 16 * __popc(w & 0xFFFF0000u);
-    A += (2 * d0 + 65) * pc - 2 * sp;
-    if (w != 0xFFFFFFFFu) s = d0 + 33 - __builtin_ctz(~w);
+    // sp in Horner form, one v_lshl_add per level (the compiler would
+    // otherwise expand it into separate shifts and adds)
+    int sp = __popc(w & 0xFFFF0000u);
+    sp = lshl1_add(sp, __popc(w & 0xFF00FF00u));
+    sp = lshl1_add(sp, __popc(w & 0xF0F0F0F0u));
+    sp = lshl1_add(sp, __popc(w & 0xCCCCCCCCu));
+    sp = lshl1_add(sp, __popc(w & 0xAAAAAAAAu));
+    A += (int)__umul24(2 * d0 + 65, pc) - 2 * sp;  // operands < 2^13
+    // s = d0 + 33 - ctz(~w) when the block has an invisible step: such a value
+    // exceeds every earlier s (<= d0 + 1).  v_ffbl gives ~0u for ~w == 0, which
+    // the saturating subtraction turns into 0, so the max keeps s.
+    uint32_t f;
+    asm("v_ffbl_b32 %0, %1" : "=v"(f) : "v"(~w));
+    s = max(s, (int)__builtin_elementwise_sub_sat((uint32_t)(d0 + 33), f));
 }
 
 // The visibility flag is the sign of t = M - q: rounding preserves the sign
@@ -375,9 +391,12 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
             if (k < ns) {
                 uint32_t w = scan_block<kDir, false>(row + jj[k], h[k], rtab, d0, 0, M[k]);
                 const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
-                // steps past the row end (the last e of the block) are recorded as visible
-                const int e = min(max(d0 + 32 - myLen, 0), 31);
-                w |= (1u << e) - 1u;
+                // steps past the row end (the last e of the block) are recorded as
+                // visible; skipped when no lane's row ends in this block
+                if (__builtin_amdgcn_ballot_w64(myLen < d0 + 32) != 0) {
+                    const int e = min(max(d0 + 32 - myLen, 0), 31);
+                    w |= (1u << e) - 1u;
+                }
                 flush_block(w, d0, A[k], s[k]);
             }
         }
