@@ -18,6 +18,7 @@ struct ske_fbp {
     ske::DevBuf<float> taps;
     std::vector<float> taps_host;
     ske::DevBuf<float> buf[4];  // image, sinogram, filtered, reconstruction
+    ske::DevBuf<float> padded;  // the projector's zero-bordered image copy
     ske::LaunchTimer timer;
     ~ske_fbp() {
         if (stream) {

@@ -23,23 +23,29 @@
 
 namespace ske {
 
-__device__ __forceinline__ float bilinear(const float* __restrict__ img, int n, float u, float v) {
+// The projector reads a copy of each image with a zero border of kProjPad
+// pixels, so the bilinear taps need no bounds tests: the samples of
+// ray_range lie within 1 + 2 delta of the centre along each axis, i.e. their
+// taps within 3 pixels of the image (checked exhaustively on the CPU for
+// n = 2..1024, 2..2n+3 bins and arbitrary angles), and zeros there equal the
+// zero-outside rule.
+__global__ __launch_bounds__(256) void k_fbp_pad(const float* __restrict__ img, float* __restrict__ pad, int n,
+                                                 int np, size_t total) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const size_t per = (size_t)np * np;
+    const size_t b = e / per;
+    const int r = (int)((e - b * per) / np) - kProjPad, c = (int)(e % np) - kProjPad;
+    pad[e] = (r >= 0 && r < n && c >= 0 && c < n) ? img[b * n * n + (size_t)r * n + c] : 0.0f;
+}
+
+__device__ __forceinline__ float bilinear_padded(const float* __restrict__ pim, int np, float u, float v) {
     // u: column coordinate, v: row coordinate, pixel centres at integers
     const float fu = floorf(u), fv = floorf(v);
-    const int j0 = (int)fu, i0 = (int)fv;
+    // 24-bit multiply: row and np are below 2^14, the offset below 2^31
+    const float* r = pim + (__mul24((int)fv + kProjPad, np) + ((int)fu + kProjPad));
     const float wu = u - fu, wv = v - fv;
-    float a = 0.0f, b = 0.0f, c = 0.0f, d = 0.0f;
-    const bool j0in = j0 >= 0 && j0 < n, j1in = j0 + 1 >= 0 && j0 + 1 < n;
-    if (i0 >= 0 && i0 < n) {
-        const float* r = img + (size_t)i0 * n;
-        if (j0in) a = r[j0];
-        if (j1in) b = r[j0 + 1];
-    }
-    if (i0 + 1 >= 0 && i0 + 1 < n) {
-        const float* r = img + (size_t)(i0 + 1) * n;
-        if (j0in) c = r[j0];
-        if (j1in) d = r[j0 + 1];
-    }
+    const float a = r[0], b = r[1], c = r[np], d = r[np + 1];
     const float top = a + wu * (b - a);
     const float bot = c + wu * (d - c);
     return top + wv * (bot - top);
@@ -76,21 +82,21 @@ __device__ __forceinline__ void ray_range(const FbpGeom& g, float s, float c, fl
     }
 }
 
-__device__ __forceinline__ float ray_sample(const float* __restrict__ im, const FbpGeom& g, float s, float c,
-                                            float sn, int i) {
+__device__ __forceinline__ float ray_sample(const float* __restrict__ pim, int np, const FbpGeom& g, float s,
+                                            float c, float sn, int i) {
     const float half = 0.5f * (float)(g.nt - 1);
     const float inv = 1.0f / g.delta;
     const float t = ((float)i - half) * g.delta;
     const float x = s * c - t * sn;
     const float y = s * sn + t * c;
-    return bilinear(im, g.n, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
+    return bilinear_padded(pim, np, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
 }
 
 // Angles with |cos| >= |sin|: one thread per (bin, angle, slice) summing its
 // ray's samples in order.  Neighbouring lanes (bins) sample points
 // delta (cos, sin) apart: nearly the same image row, so the gathers
 // coalesce.  `ang` lists the angle indices this launch covers.
-__global__ __launch_bounds__(256) void k_fbp_project_bins(const float* __restrict__ img, float* __restrict__ sino,
+__global__ __launch_bounds__(256) void k_fbp_project_bins(const float* __restrict__ pimg, float* __restrict__ sino,
                                                           const float2* __restrict__ cs, const int* __restrict__ ang,
                                                           FbpGeom g) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,16 +107,28 @@ __global__ __launch_bounds__(256) void k_fbp_project_bins(const float* __restric
     const float s = ((float)k - g.det_c) * g.delta;
     int i0, i1;
     ray_range(g, s, t2.x, t2.y, i0, i1);
-    const float* im = img + (size_t)b * g.n * g.n;
+    const int np = g.n + 2 * kProjPad;
+    const float* im = pimg + (size_t)b * np * np;
+    // four samples' gathers in flight per pass, summed in sample order
     float acc = 0.0f;
-    for (int i = i0; i <= i1; ++i) acc += ray_sample(im, g, s, t2.x, t2.y, i);
+    int i = i0;
+    for (; i + 3 <= i1; i += 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ray_sample(im, np, g, s, t2.x, t2.y, i + u);
+        acc += v[0];
+        acc += v[1];
+        acc += v[2];
+        acc += v[3];
+    }
+    for (; i <= i1; ++i) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
     sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
 }
 
 // Angles with |sin| > |cos|: one wave per ray, lanes over its samples
 // (neighbouring samples are delta (-sin, cos) apart: again nearly one image
 // row), then a wave reduction.
-__global__ __launch_bounds__(256) void k_fbp_project_rays(const float* __restrict__ img, float* __restrict__ sino,
+__global__ __launch_bounds__(256) void k_fbp_project_rays(const float* __restrict__ pimg, float* __restrict__ sino,
                                                           const float2* __restrict__ cs, const int* __restrict__ ang,
                                                           FbpGeom g) {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -122,9 +140,17 @@ __global__ __launch_bounds__(256) void k_fbp_project_rays(const float* __restric
     const float s = ((float)k - g.det_c) * g.delta;
     int i0, i1;
     ray_range(g, s, t2.x, t2.y, i0, i1);
-    const float* im = img + (size_t)b * g.n * g.n;
+    const int np = g.n + 2 * kProjPad;
+    const float* im = pimg + (size_t)b * np * np;
     float acc = 0.0f;
-    for (int i = i0 + lane; i <= i1; i += 64) acc += ray_sample(im, g, s, t2.x, t2.y, i);
+    int i = i0 + lane;
+    for (; i + 64 <= i1; i += 128) {
+        const float v0 = ray_sample(im, np, g, s, t2.x, t2.y, i);
+        const float v1 = ray_sample(im, np, g, s, t2.x, t2.y, i + 64);
+        acc += v0;
+        acc += v1;
+    }
+    if (i <= i1) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (lane == 0) sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
@@ -231,14 +257,17 @@ __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict
     if (in) img[(size_t)b * g.n * g.n + (size_t)i * g.n + j] = acc * scale;
 }
 
-hipError_t launch_fbp_project(const float* img, float* sino, const float2* cs, const int* ang_bins, int n_bins,
-                              const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s) {
+hipError_t launch_fbp_project(const float* img, float* padded, float* sino, const float2* cs, const int* ang_bins,
+                              int n_bins, const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s) {
+    const int np = g.n + 2 * kProjPad;
+    const size_t total = (size_t)batch * np * np;
+    hipLaunchKernelGGL(k_fbp_pad, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, padded, g.n, np, total);
     if (n_bins > 0)
-        hipLaunchKernelGGL(k_fbp_project_bins, dim3((g.ndet + 255) / 256, n_bins, batch), dim3(256), 0, s, img, sino,
-                           cs, ang_bins, g);
+        hipLaunchKernelGGL(k_fbp_project_bins, dim3((g.ndet + 255) / 256, n_bins, batch), dim3(256), 0, s, padded,
+                           sino, cs, ang_bins, g);
     if (n_rays > 0)
-        hipLaunchKernelGGL(k_fbp_project_rays, dim3((g.ndet + 3) / 4, n_rays, batch), dim3(256), 0, s, img, sino, cs,
-                           ang_rays, g);
+        hipLaunchKernelGGL(k_fbp_project_rays, dim3((g.ndet + 3) / 4, n_rays, batch), dim3(256), 0, s, padded, sino,
+                           cs, ang_rays, g);
     return hipGetLastError();
 }
 
