@@ -197,6 +197,11 @@ def extra_c2_radon(ske, synth):
         t = _timed(lambda: e.run(ske.KERNEL_RADON, copy_out=False), e.synchronize, 20)
     res["skew_radon_fbp"] = {"workload": "512x512 phantom, 180 skew sectors (kernelRADN semantics)",
                              "ms_per_slice": t * 1e3, "slices_per_s": 1.0 / t}
+    nb = 16
+    with ske.SkewBatch([synth.ellipse_phantom(512, seed=200 + i) for i in range(nb)]) as b:
+        t = _timed(lambda: b.run(ske.KERNEL_RADON), b.synchronize, 10)
+    res["skew_radon_fbp_batch"] = {"workload": "%d x 512x512 phantoms, 180 skew sectors each, one batched run" % nb,
+                                   "ms_per_slice": t * 1e3 / nb, "slices_per_s": nb / t}
     from skewengine_amd import _lib
     cells, _ = ske.work(512, 512)
     with ske.SkewEngine(img) as e:

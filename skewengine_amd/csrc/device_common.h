@@ -26,6 +26,19 @@ __device__ __forceinline__ float skew_cell(const float* __restrict__ src, int t,
     return s;
 }
 
+// skew_cell with both taps read unconditionally (row indices clamped into
+// the image) and masked afterwards, so several cells' gathers can be in
+// flight at once; the same value bit for bit.
+__device__ __forceinline__ float skew_cell_sel(const float* __restrict__ src, int t, float w,
+                                               int r, int j, int dim_i, int dim_o) {
+    const int i1 = r - t - 1;
+    const int i0 = r - t;
+    const float x1 = src[(size_t)min(max(i1, 0), dim_o - 1) * dim_i + j];
+    const float x0 = src[(size_t)min(max(i0, 0), dim_o - 1) * dim_i + j];
+    const float s = (i1 >= 0 && i1 < dim_o) ? 0.0f + w * x1 : 0.0f;
+    return (i0 >= 0 && i0 < dim_o) ? s + (1.0f - w) * x0 : s;
+}
+
 __device__ __forceinline__ int fl_first(uint32_t v) { return (int)(v & 0xffffu); }
 __device__ __forceinline__ int fl_last(uint32_t v) { return (int)(v >> 16); }
 

@@ -35,7 +35,26 @@ __global__ __launch_bounds__(256) void k_radn_rowsum(const SecDesc* __restrict__
     const int f = fl_first(v), l = fl_last(v);
     const float* src = in4 + sd.img + (size_t)sd.type * sd.N;
     double acc = 0.0;
-    for (int j = f + lane; j < l; j += 64) {
+    int j = f + lane;
+    if (!Sbuf) {
+        // four cells per lane and pass: table reads, then gathers, in flight
+        // together; the lane's sum still runs over j in ascending order
+        for (; j + 192 < l; j += 256) {
+            int t[4];
+            float wt[4], x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                t[u] = target[sd.tab + j + 64 * u];
+                wt[u] = weight[sd.tab + j + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                x[u] = skew_cell_sel(src, t[u], wt[u], w.row, j + 64 * u, sd.dim_i, sd.dim_o);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += (double)x[u];
+        }
+    }
+    for (; j < l; j += 64) {
         float x;
         if (Sbuf)
             x = Sbuf[(size_t)w.row * sd.dim_i + j];
