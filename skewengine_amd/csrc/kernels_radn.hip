@@ -213,8 +213,8 @@ __global__ __launch_bounds__(128) void k_lrt_rowsum(const SecDesc* __restrict__ 
     }
 }
 
-// tab: cos/sin(2 pi t/64) interleaved at [0,128), the sine ramp
-// sin(f/24.6 * pi/2) for f <= 25 at [160,186); gauss: 64 weights per folded
+// tab: the sine ramp sin(f/24.6 * pi/2) for f <= 25 at [160,186), the two
+// DFT matrices at kLrtC and kLrtY (ske_types.h); gauss: 64 weights per folded
 // angle (newAngle).
 __global__ __launch_bounds__(64) void k_op_lrt(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
                                                const uint32_t* __restrict__ fl, const double* __restrict__ W,
@@ -232,21 +232,36 @@ __global__ __launch_bounds__(64) void k_op_lrt(const SecDesc* __restrict__ secs,
         double re1[26];
 #pragma unroll
         for (int q = 0; q < 26; ++q) re1[q] = 0.0;
-        for (int kk = k0; kk < k1; ++kk) {
-            const double x = Wb[(size_t)(i - 32 + kk) * sd.dim_i + j] * g[kk];
+        int kk = k0;
+        // four window sums in flight per pass; re1 still accumulates in kk order
+        for (; kk + 4 <= k1; kk += 4) {
+            double xs[4];
 #pragma unroll
-            for (int q = 0; q < 26; ++q) re1[q] = __builtin_fma(x, tab[2 * ((q * kk) & 63)], re1[q]);
+            for (int u = 0; u < 4; ++u) xs[u] = Wb[(size_t)(i - 32 + kk + u) * sd.dim_i + j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double x = xs[u] * g[kk + u];
+                const double* C = tab + kLrtC + 32 * (kk + u);  // wave-uniform row: scalar loads
+#pragma unroll
+                for (int q = 0; q < 26; ++q) re1[q] = __builtin_fma(x, C[q], re1[q]);
+            }
+        }
+        for (; kk < k1; ++kk) {
+            const double x = Wb[(size_t)(i - 32 + kk) * sd.dim_i + j] * g[kk];
+            const double* C = tab + kLrtC + 32 * kk;
+#pragma unroll
+            for (int q = 0; q < 26; ++q) re1[q] = __builtin_fma(x, C[q], re1[q]);
         }
 #pragma unroll
         for (int q = 0; q < 26; ++q) re1[q] *= 0.125 * tab[160 + q];
         double prev = 0.0, val = 0.0;
         for (int m = 4; m < 32; ++m) {
             double yr = 0.0, yi = 0.0;
+            const double* Y = tab + kLrtY + 64 * m;
 #pragma unroll
             for (int q = 0; q < 26; ++q) {
-                const int t = (q * m) & 63;
-                yr = __builtin_fma(re1[q], tab[2 * t], yr);
-                yi = __builtin_fma(re1[q], tab[2 * t + 1], yi);
+                yr = __builtin_fma(re1[q], Y[2 * q], yr);
+                yi = __builtin_fma(re1[q], Y[2 * q + 1], yi);
             }
             yr *= 0.125;
             yi *= 0.125;

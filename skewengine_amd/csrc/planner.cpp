@@ -198,7 +198,8 @@ void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off) {
 }
 
 void cepstrum_table(std::vector<double>& tab) {
-    tab.assign(kLogTab + 2 * 256 + 2, 0.0);
+    static_assert(kLrtC >= kLogTab + 2 * 256 + 2, "table layout");
+    tab.assign(kCtabSize, 0.0);
     for (int t = 0; t < 64; t++) {
         tab[2 * t] = cos(2.0 * M_PI * t / 64.0);
         tab[2 * t + 1] = sin(2.0 * M_PI * t / 64.0);
@@ -217,6 +218,16 @@ void cepstrum_table(std::vector<double>& tab) {
     const long double ln2 = logl(2.0L);
     tab[kLogTab + 512] = (double)ln2;
     tab[kLogTab + 513] = (double)(ln2 - (long double)(double)ln2);
+    // the local Radon's two DFT matrices, rows contiguous (copies of the
+    // cos/sin entries above; frequencies q >= 26 stay 0)
+    for (int kk = 0; kk < 64; kk++)
+        for (int q = 0; q < 26; q++) tab[kLrtC + 32 * kk + q] = tab[2 * ((q * kk) & 63)];
+    for (int m = 0; m < 32; m++)
+        for (int q = 0; q < 26; q++) {
+            const int t = (q * m) & 63;
+            tab[kLrtY + 64 * m + 2 * q] = tab[2 * t];
+            tab[kLrtY + 64 * m + 2 * q + 1] = tab[2 * t + 1];
+        }
 }
 
 void lrt_gauss_table(std::vector<double>& tab) {
