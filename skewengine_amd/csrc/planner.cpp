@@ -72,20 +72,26 @@ void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_
 
 void plan_finish(ImagePlan& ip, int order) {
     // ip.work holds rows in natural order; sort by descending cost so that a
-    // dynamic scheduler hands out the long rows first.
-    std::vector<std::pair<double, RowWork>> rows;
+    // dynamic scheduler hands out the long rows first.  kOrderNatural keeps
+    // the images of a batch apart (image by image, each by descending L), so
+    // the rows in flight read one image's oriented copies at a time.
+    struct Key {
+        long long img;
+        double cost;
+        RowWork w;
+    };
+    std::vector<Key> rows;
     rows.reserve(ip.work.size());
     for (const RowWork& w : ip.work) {
         const SecDesc& sd = ip.secs[w.sec];
         const uint32_t v = ip.fl[sd.rows + w.row];
         const double L = (double)((int)(v >> 16) - (int)(v & 0xffffu));
-        rows.push_back({order == kOrderCostDesc ? L * L : L, w});
+        rows.push_back({order == kOrderCostDesc ? 0 : sd.img, order == kOrderCostDesc ? L * L : L, w});
     }
-    std::stable_sort(rows.begin(), rows.end(),
-                     [](const std::pair<double, RowWork>& x, const std::pair<double, RowWork>& y) {
-                         return x.first > y.first;
-                     });
-    for (size_t i = 0; i < rows.size(); i++) ip.work[i] = rows[i].second;
+    std::stable_sort(rows.begin(), rows.end(), [](const Key& x, const Key& y) {
+        return x.img != y.img ? x.img < y.img : x.cost > y.cost;
+    });
+    for (size_t i = 0; i < rows.size(); i++) ip.work[i] = rows[i].w;
 }
 
 void plan_append(int dimx, int dimy, int s0, int s1, long long img, ImagePlan& ip, int range8[8]) {

@@ -48,7 +48,8 @@ enum WorkOrder { kOrderNatural = 0, kOrderCostDesc = 1 };
 // Plan sectors [s0, s1) of a dimx x dimy image.  `img` is the element offset
 // of the image's four oriented copies; `so_base`/`q_base` offset its arena
 // and projection storage.  Rows are sorted by descending L^2 cost when
-// `order` is kOrderCostDesc (viewshed), by descending L otherwise.
+// `order` is kOrderCostDesc (viewshed), otherwise image by image and by
+// descending L within an image.
 void plan_image(int dimx, int dimy, int s0, int s1, long long img, long long so_base, long long q_base,
                 int order, ImagePlan& ip);
 
