@@ -28,11 +28,12 @@ sys.path.insert(0, ROOT)
 # of v_sub/v_max, consistent with that figure.
 PEAK_VALU_TOPS = 157.3 / 2
 OPS_PER_STEP = 4  # sub, divide, compare, max per ray step (src/kernelSDEM.h:191-196)
-# VALU instructions per executed lane-step of the compacting bit-block scan
-# loop in the gfx950 build: about 270 per 32-step block of one slot (6 per
-# step, then the popcount flush, the end-of-row mask, the early-exit bound
-# and the retire/compaction bookkeeping).
-VALU_PER_LANE_STEP = 270 / 32
+# VALU wave instructions per 64 executed lane-steps of the viewshed kernel,
+# measured with rocprofv3 --pmc SQ_INSTS_VALU over the bench's timed step
+# (6 per step in the scan, then per 32-step block of a slot the popcount
+# flush, the end-of-row mask, the early-exit bound and the retire/compaction
+# bookkeeping, plus row staging and chunk tables).
+VALU_PER_LANE_STEP = 8.72
 HBM_PEAK_GBS = 8000.0
 
 
