@@ -198,7 +198,9 @@ typedef struct ske_fbp ske_fbp;
 #define SKE_FBP_SINO 1     /* batch x n_angles x n_det */
 #define SKE_FBP_FILTERED 2 /* batch x n_angles x n_det */
 #define SKE_FBP_RECON 3    /* batch x n x n */
-/* angles: n_angles radians, or NULL for a*pi/n_angles, a = 0..n_angles-1. */
+/* angles: n_angles radians, or NULL for a*pi/n_angles, a = 0..n_angles-1.
+ * Device memory: the four buffers above for max_batch slices, plus the
+ * projector's zero-bordered image copy, max_batch x (n+8)^2 floats. */
 int ske_fbp_create(int n, int n_angles, int n_det, const float *angles, int max_batch, int device_id,
                    ske_fbp **out);
 int ske_fbp_destroy(ske_fbp *f);
