@@ -291,32 +291,44 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
     mid_layer<kEdge>(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
     if (kTrain) store_centre(sa, kF3, 1, a3g + img_off, i0, j0, n);
-    // conv 16 -> 1 + bias + residual: one thread per pixel and channel half
+    // conv 16 -> 1 + bias + residual: one thread per horizontal pixel pair and
+    // channel half.  The pair's 3 x 4 neighbourhood is read once (12 LDS reads
+    // instead of 2 x 9); each pixel still sums its taps 0..8 in order.
     float dsum = 0.0f;  // training: this thread's squared errors
-    for (int t = threadIdx.x; t < 2 * kFT * kFT; t += 64 * kFWaves) {
-        const int px = t >> 1, part = t & 1;
-        const int y = px / kFT, x = px % kFT;
-        float acc = 0.0f;
+    for (int t = threadIdx.x; t < kFT * kFT; t += 64 * kFWaves) {
+        const int pr = t >> 1, part = t & 1;
+        const int y = pr / (kFT / 2), x = 2 * (pr % (kFT / 2));
+        float acc0 = 0.0f, acc1 = 0.0f;
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const uint4 v = *(const uint4*)(sa + ((y + tap / 3) * kF3 + x + tap % 3) * kP + 8 * part);
-            const uint32_t* wp = w4 + tap * (C / 2) + 4 * part;
-            acc = dot2_bf16(wp[0], v.x, acc);
-            acc = dot2_bf16(wp[1], v.y, acc);
-            acc = dot2_bf16(wp[2], v.z, acc);
-            acc = dot2_bf16(wp[3], v.w, acc);
-        }
-        const float other = __shfl_xor(acc, 1, 64);
-        const float s0v = part == 0 ? acc : other, s1v = part == 0 ? other : acc;
-        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + x;
-        const float o = img[(size_t)(i0 + y) * n + j0 + x] + ((s0v + s1v) + prm[kB4]);
-        if (part == 0) {
-            out[p] = o;
-            if (kTrain && target) {
-                const float d = o - target[p];
-                dout[p] = 2.0f * d * inv_count;
-                dsum += d * d;
+        for (int ty = 0; ty < 3; ty++) {
+            uint4 c[4];
+#pragma unroll
+            for (int cx = 0; cx < 4; cx++) c[cx] = *(const uint4*)(sa + ((y + ty) * kF3 + x + cx) * kP + 8 * part);
+#pragma unroll
+            for (int tx = 0; tx < 3; tx++) {
+                const uint32_t* wp = w4 + (3 * ty + tx) * (C / 2) + 4 * part;
+                acc0 = dot2_bf16(wp[0], c[tx].x, acc0);
+                acc0 = dot2_bf16(wp[1], c[tx].y, acc0);
+                acc0 = dot2_bf16(wp[2], c[tx].z, acc0);
+                acc0 = dot2_bf16(wp[3], c[tx].w, acc0);
+                acc1 = dot2_bf16(wp[0], c[tx + 1].x, acc1);
+                acc1 = dot2_bf16(wp[1], c[tx + 1].y, acc1);
+                acc1 = dot2_bf16(wp[2], c[tx + 1].z, acc1);
+                acc1 = dot2_bf16(wp[3], c[tx + 1].w, acc1);
             }
+        }
+        // the other half's sums; the part-0 thread finishes pixel x, part 1 pixel x + 1
+        const float o0 = __shfl_xor(acc0, 1, 64), o1 = __shfl_xor(acc1, 1, 64);
+        const float s0v = part == 0 ? acc0 : o1;  // channel half 0 of this thread's pixel
+        const float s1v = part == 0 ? o0 : acc1;  // channel half 1
+        const int xx = x + part;
+        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + xx;
+        const float o = img[(size_t)(i0 + y) * n + j0 + xx] + ((s0v + s1v) + prm[kB4]);
+        out[p] = o;
+        if (kTrain && target) {
+            const float d = o - target[p];
+            dout[p] = 2.0f * d * inv_count;
+            dsum += d * d;
         }
     }
     if (kTrain && target) {  // uniform
