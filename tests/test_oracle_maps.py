@@ -151,8 +151,9 @@ def test_reduce_inverts_prepare():
         assert np.array_equal(out.reshape(37, 53), img), k
 
 
-def test_work_counts_match_independent_survey_numbers():
-    # Appendix-C counts of an independent restatement of the maps.
+def test_work_counts_match_independent_restatement():
+    # Skewed-cell and ray-step counts computed by an independent restatement
+    # of the maps (written separately from the oracle).
     assert oracle.work(500, 500) == pytest.approx((45068242, 1.9179e10), rel=1e-4)
     c, s = oracle.work(512, 512)
     assert c == 47255794
