@@ -89,7 +89,7 @@ def test_per_sector_driver_matches_oracle_execute(kernel, dem64, img64):
         assert np.mean(out != ref) <= 1e-3
 
 
-@pytest.mark.parametrize("n", [64, 128])
+@pytest.mark.parametrize("n", [50, 64, 70, 128])  # 50, 70: the deskew's last rows are a partial block of 16
 @pytest.mark.parametrize("kernel", [oracle.SDEM, oracle.IDENTITYDEM, oracle.IDENTITYIMG])
 def test_run_all_sectors_bit_exact(kernel, n):
     grid = synth.dem(n, 0.8, n, 1500.0)
