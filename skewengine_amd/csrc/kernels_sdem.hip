@@ -127,19 +127,26 @@ __global__ __launch_bounds__(256) void k_op_sdem(const SecDesc* __restrict__ sec
 // the float cv of the reference (whose partial sums are exact integers
 // there) is reproduced bit for bit.
 // ---------------------------------------------------------------------------
-constexpr int kPadBits = 128;
+// Guard band on each side of a staged row: the one-chunk scans of a chunk
+// run to its first lane's row end, up to 95 cells past the row (128); with
+// lane compaction a block only runs while some lane's row continues, so
+// every read stays within 32 cells of the row (and of the safe positions).
+template <int kG>
+constexpr int pad_bits() { return kG > 1 ? 32 : 128; }
 constexpr int kChunksMax = 128;  // 32-cell chunks of a row of <= 4096 cells
 // Chunk-maximum tables of a staged row, each indexed by chunk c in
 // [-kTabLo, kChunksMax + kTabHi): level 0 cm[c] = max row[32c .. 32c+31],
 // level 1..3 the maxima of 2, 4 and 8 consecutive chunks starting at c,
 // then suffix and prefix maxima.  Chunks outside the row hold -inf (the
 // prefix table repeats the row maximum past the end).
-constexpr int kTabLo = 16, kTabHi = 32;
+constexpr int kTabLo = 16, kTabHi = 16;
 constexpr int kTabN = kTabLo + kChunksMax + kTabHi;
 enum { kTabL0 = 0, kTabL1 = 1, kTabL2 = 2, kTabL3 = 3, kTabSuf = 4, kTabPre = 5, kTabs = 6 };
 
 // LDS of the bit-block kernels: guard band, row, guard band, chunk tables.
-inline size_t sdem_bits_lds(int maxL) { return sizeof(float) * ((size_t)maxL + 2 * kPadBits + kTabs * kTabN); }
+inline size_t sdem_bits_lds(int maxL, int group = 1) {
+    return sizeof(float) * ((size_t)maxL + 2 * (group > 1 ? pad_bits<4>() : pad_bits<1>()) + kTabs * kTabN);
+}
 
 __device__ __forceinline__ int lshl1_add(int a, int b) {
     int r;
@@ -505,10 +512,10 @@ __device__ __forceinline__ void sdem_bits_row(const RowWork w, const SecDesc* __
     const int f = fl_first(v);
     const int L = fl_last(v) - f;
     if (L <= 0) return;  // uniform; the work list holds only rows with L > 0
-    stage_row(row, sd, r, f, L, in4, target, weight, Sbuf, kPadBits);
+    stage_row(row, sd, r, f, L, in4, target, weight, Sbuf, pad_bits<kG>());
     if (kG > 1 && threadIdx.x == 0) *gnext = kWaves;
     __syncthreads();
-    float* tab = row + L + kPadBits;  // past the right guard band
+    float* tab = row + L + pad_bits<kG>();  // past the right guard band
     row_tables<kWaves>(row, L, tab);
 
     const int lane = threadIdx.x & 63;
@@ -563,7 +570,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_op_sdem_bits(const SecDesc* __r
     extern __shared__ float lds[];
     __shared__ int gnext;
     sdem_bits_row<kWaves, kG>(work[blockIdx.x], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
-                              lds + kPadBits, steps, &gnext);
+                              lds + pad_bits<kG>(), steps, &gnext);
 }
 
 // Persistent: a fixed grid pulls rows off the (cost-sorted) work list with
@@ -586,7 +593,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kOc
         __syncthreads();  // every lane has read `item` before thread 0 overwrites it
         if (it >= nwork) break;
         sdem_bits_row<kWaves, kG>(work[it], secs, in4, target, weight, fl, Sbuf, rtab, povh, arena,
-                                  lds + kPadBits, steps, &gnext);
+                                  lds + pad_bits<kG>(), steps, &gnext);
     }
 }
 
@@ -601,7 +608,7 @@ hipError_t launch_op_sdem_tuned(const DevPlan& p, const float* in4, const float*
     if (p.nwork == 0) return hipSuccess;
     if (p.maxL > 4096) return launch_op_sdem(p, in4, Sbuf, arena, rtab, povh, 2, s);
     const int G = t.group > 1 ? 4 : 1;
-    const size_t lds = sdem_bits_lds(p.maxL);
+    const size_t lds = sdem_bits_lds(p.maxL, G);
     if (t.persistent_wgs_per_cu > 0 && counter) {
         int dev = 0, ncu = 0;
         hipError_t e = hipGetDevice(&dev);
