@@ -130,7 +130,8 @@ int ske_synchronize(ske_engine *e);
 #define SKE_OPT_ARENA_BYTES 2   /* device budget for one chunk of skewOutput rows */
 #define SKE_OPT_PROFILE 3       /* 1: time every launch with HIP events on the engine stream */
 #define SKE_OPT_GRAPH 4         /* 1: capture the all-sector run once in a hipGraph and replay it */
-#define SKE_OPT_SDEM_WAVES 5    /* viewshed: waves per workgroup (2, 4, 8) */
+#define SKE_OPT_SDEM_WAVES 5    /* viewshed: waves per workgroup (2, 4, 8); unset: 4, or 2 for rows of
+                                 * fewer than four chunk groups (or chunks) per scan direction */
 #define SKE_OPT_SDEM_PERSISTENT 6 /* viewshed: 0 one workgroup per row, k>0 persistent grid of k per CU */
 #define SKE_OPT_SDEM_OCCUPANCY 7  /* viewshed persistent grid: 0 default, 8 compiled for 8 waves/SIMD */
 #define SKE_OPT_RADON_LOCAL 8     /* 1: SKE_KERNEL_RADON computes the local transform (pointRadon per

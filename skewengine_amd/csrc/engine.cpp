@@ -143,6 +143,7 @@ struct Core {
     long long arena_bytes = 48LL << 30;
     bool profile = false;
     bool use_graph = false;
+    bool waves_set = false;  // waves per workgroup chosen by the caller or the autotuner
     bool count_steps = false;  // viewshed: count executed lane-steps (counter[2..3] as a u64)
     std::map<std::string, double> counters;
     // profiling
@@ -384,6 +385,20 @@ struct Core {
         return SKE_OK;
     }
 
+    // Without a caller's or the autotuner's choice, a grid workgroup whose
+    // rows hold fewer than four work units (chunk groups, or chunks) per scan
+    // direction runs 2 waves instead of 4, so that no wave of it idles
+    // (500 x 500 grid: 2.36 instead of 3.19 ms).
+    SdemTuning effective_tuning(int maxL) const {
+        SdemTuning t = tune;
+        if (!waves_set && t.persistent_wgs_per_cu == 0) {
+            const int chunks = (maxL + 63) / 64;
+            const int units = t.group > 1 ? (chunks + t.group - 1) / t.group : chunks;
+            t.waves = units < 4 ? 2 : 4;
+        }
+        return t;
+    }
+
     // Enqueue the whole all-sector run (no allocation, no synchronisation:
     // safe to capture).
     int enqueue_run(int kernel_id, float povh) {
@@ -398,8 +413,9 @@ struct Core {
             switch (kernel_id) {
             case SKE_KERNEL_SDEM:
                 if (sdem_variant == 0) {
+                    const SdemTuning t = effective_tuning(v.maxL);
                     SKE_TRY(launch("op_sdem", [&] {
-                        return launch_op_sdem_tuned(v, in4.p, nullptr, arena.p, rtab.p, povh, tune, counter.p,
+                        return launch_op_sdem_tuned(v, in4.p, nullptr, arena.p, rtab.p, povh, t, counter.p,
                                                     count_steps ? step_counter() : nullptr, stream);
                     }));
                 } else {
@@ -501,6 +517,7 @@ struct Core {
         case SKE_OPT_SDEM_WAVES:
             if (value != 2 && value != 4 && value != 8) return fail(SKE_EINVAL, "waves per workgroup must be 2, 4 or 8");
             tune.waves = (int)value;
+            waves_set = true;
             return SKE_OK;
         case SKE_OPT_SDEM_PERSISTENT:
             if (value < 0 || value > 64) return fail(SKE_EINVAL, "persistent workgroups per CU must be in [0,64]");
@@ -603,6 +620,7 @@ struct Core {
         event_pool.push_back(b);
         SKE_HIP(hipStreamSynchronize(stream));
         tune = pick;
+        waves_set = true;
         drop_graph();
         if (waves) *waves = pick.waves;
         if (wgs) *wgs = pick.persistent_wgs_per_cu;
