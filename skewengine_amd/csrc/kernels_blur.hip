@@ -42,7 +42,7 @@ __device__ __forceinline__ double fast_log(double x, const double* lt) {
 }
 
 // tab layout (double): [0,128): cos/sin(2 pi t/64) interleaved; [128,160): cos(2 pi t/32);
-// from kLogTab the log table of fast_log.
+// from kLogTab the log table of fast_log; from kBlurD the window DFT by rows.
 __global__ __launch_bounds__(64) void k_op_blur(const SecDesc* __restrict__ secs, const RowWork* __restrict__ work,
                                                 const float* __restrict__ in4, const int* __restrict__ target,
                                                 const float* __restrict__ weight, const uint32_t* __restrict__ fl,
@@ -74,11 +74,11 @@ __global__ __launch_bounds__(64) void k_op_blur(const SecDesc* __restrict__ secs
     for (int k = 0; k < 32; ++k) { Xr[k] = 0.0; Xi[k] = 0.0; }
     for (int m = 0; m < 64; ++m) {
         const double x = (double)row[c0 - 32 + m] * 0.125;
+        const double* D = tab + kBlurD + 64 * m;  // wave-uniform row: contiguous scalar loads
 #pragma unroll
         for (int k = 0; k < 32; ++k) {
-            const int t = (k * m) & 63;
-            Xr[k] = __builtin_fma(x, tab[2 * t], Xr[k]);
-            Xi[k] = __builtin_fma(-x, tab[2 * t + 1], Xi[k]);
+            Xr[k] = __builtin_fma(x, D[2 * k], Xr[k]);
+            Xi[k] = __builtin_fma(-x, D[2 * k + 1], Xi[k]);
         }
     }
     const double inv_sqrt32 = 0.17677669529663687;  // 1/sqrt(32)

@@ -234,6 +234,13 @@ void cepstrum_table(std::vector<double>& tab) {
             tab[kLrtY + 64 * m + 2 * q] = tab[2 * t];
             tab[kLrtY + 64 * m + 2 * q + 1] = tab[2 * t + 1];
         }
+    // the cepstrum's 64-sample window DFT (bins k < 32), rows contiguous
+    for (int m = 0; m < 64; m++)
+        for (int k = 0; k < 32; k++) {
+            const int t = (k * m) & 63;
+            tab[kBlurD + 64 * m + 2 * k] = tab[2 * t];
+            tab[kBlurD + 64 * m + 2 * k + 1] = tab[2 * t + 1];
+        }
 }
 
 void lrt_gauss_table(std::vector<double>& tab) {

@@ -74,7 +74,8 @@ void fft_twiddles(int mmax, std::vector<double>& tw, std::vector<int>& off);
 // interleaved, [128,160) cos(2 pi t/32), [160,186) the local Radon's sine
 // ramp sin(k/24.6 * pi/2), then from kLogTab (ske_types.h) the natural-log
 // table of the cepstrum kernel (256 pairs c_i, -log c_i, then ln 2 as a
-// double-double), then from kLrtC and kLrtY the local Radon's DFT matrices.
+// double-double), then from kLrtC and kLrtY the local Radon's DFT matrices
+// and from kBlurD the cepstrum's window DFT matrix.
 void cepstrum_table(std::vector<double>& tab);
 
 // Gaussian window of the local Radon transform, 64 weights per folded angle

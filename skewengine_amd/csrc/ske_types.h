@@ -13,7 +13,8 @@ constexpr int kPad = 64;        // LDS guard band on each side of a staged row
 constexpr int kLogTab = 192;    // offset of the log table in the cepstrum constants (planner.cpp)
 constexpr int kLrtC = 720;      // local Radon first DFT: cos(2 pi (q kk mod 64)/64) at [kLrtC + 32 kk + q]
 constexpr int kLrtY = kLrtC + 2048;  // second DFT: cos, sin of 2 pi (q m mod 64)/64 at [kLrtY + 64 m + 2 q (+1)]
-constexpr int kCtabSize = kLrtY + 2048;
+constexpr int kBlurD = kLrtY + 2048;  // cepstrum window DFT: cos, sin of 2 pi (k m mod 64)/64 at [kBlurD + 64 m + 2 k (+1)]
+constexpr int kCtabSize = kBlurD + 4096;
 constexpr int kNumSectors = 180; // maxSector, src/global.cpp:69
 
 // One sector of one image, as the device sees it.  Mirrors the public fields
