@@ -111,17 +111,16 @@ __device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
 // rows y, y+1 (kb = 0) or y+2 (kb = 1); `se`/`so` hold the pairs starting at
 // even / odd columns.
 __device__ __forceinline__ bf16x8 in_operand(const uint32_t* se, const uint32_t* so, int y, int x, int kb) {
+    // branch-free: every lane reads rows y + 2*(kb == 1) and y + 1 (both
+    // inside the 24-row tile for y < 22) and zeroes what its k block lacks
     const uint32_t* t = ((x & 1) ? so : se) + (x >> 1);
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (kb == 0) {
-        v.x = t[y * kS0];
-        v.y = t[y * kS0 + 1];
-        v.z = t[(y + 1) * kS0];
-        v.w = t[(y + 1) * kS0 + 1];
-    } else if (kb == 1) {
-        v.x = t[(y + 2) * kS0];
-        v.y = t[(y + 2) * kS0 + 1];
-    }
+    const int ra = (y + (kb == 1 ? 2 : 0)) * kS0, rb = (y + 1) * kS0;
+    const uint32_t a0 = t[ra], a1 = t[ra + 1], b0 = t[rb], b1 = t[rb + 1];
+    uint4 v;
+    v.x = kb < 2 ? a0 : 0u;
+    v.y = kb < 2 ? a1 : 0u;
+    v.z = kb == 0 ? b0 : 0u;
+    v.w = kb == 0 ? b1 : 0u;
     return as_bf16x8(v);
 }
 
