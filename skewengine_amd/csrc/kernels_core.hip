@@ -276,8 +276,9 @@ __global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs
 }
 
 // 16 rows, one sector per pass: measured fastest on the batched viewshed
-// (129 VGPRs; 8 rows 1.2x, 4 rows 1.9x, 1 row 4.1x slower; a second sector per
-// pass only adds registers).
+// (129 VGPRs; 8 rows 1.2x, 4 rows 1.9x, 1 row 4.1x slower, 24 or 32 rows
+// 1.45x slower at 151 / 191 VGPRs; a second sector per pass only adds
+// registers).
 constexpr int kDeskewRows = 16, kDeskewU = 1;
 
 hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source, int is_max,
