@@ -67,29 +67,50 @@ class skewEngine {
         skewOutput = new T[(size_t)N * 2 + dim_l]();
     }
 
+    // computeParams(angle) (:503-539): the public per-sector fields and the
+    // target/weight mirrors (allocated by skewAlloc if needed), on the host.
+    void computeParams(int angle) {
+        skewAlloc();
+        ske_sector_params p;
+        check(ske_plan_sector(dimx, dimy, angle, &p, target, weight, nullptr, nullptr));
+        set_fields(p);
+        planned_ = angle;
+    }
+    // skewShape() (:572-596): the first/last row extents of the sector of
+    // the last computeParams.
+    void skewShape() {
+        if (planned_ < 0) throw std::runtime_error("skewEngine: skewShape() before computeParams()");
+        check(ske_plan_sector(dimx, dimy, planned_, nullptr, nullptr, nullptr, first, last));
+    }
+    // resetSkew() (:541-570): zero skewInput/skewOutput over skewHeight*dim_i
+    // cells, in the host mirrors and, once a sector is skewed, on the device.
+    void resetSkew() {
+        skewAlloc();
+        const size_t cells = (size_t)skewHeight * dim_i;
+        memset(skewInput, 0, cells * sizeof(T));
+        memset(skewOutput, 0, cells * sizeof(T));
+        if (skewed_) check(ske_reset_skew(e_));
+    }
+
     // skew(angle) (:605): computeParams + skewShape + resetSkew + skew; the
     // public fields and (after skewAlloc) the table mirrors are updated.
     void skew(int angle) {
         check(ske_skew(e_, angle));
+        skewed_ = true;
+        planned_ = angle;
         ske_sector_params p;
         check(ske_get_params(e_, &p));
-        a = p.a;
-        newAngle = p.newAngle;
-        sectorType = p.sectorType;
-        dim_i = p.dim_i;
-        dim_o = p.dim_o;
-        offset = p.offset;
-        skewHeight = p.skewHeight;
-        skewWidth = p.skewWidth;
-        scale = p.scale;
-        skewness = p.skewness;
+        set_fields(p);
         if (target) check(ske_get_tables(e_, target, weight, first, last));
     }
 
     // kernel() (:694).
     void kernel() {
         if (kernelcpu) {
-            skewAlloc();
+            if (!target) {  // first use of the mirrors: fill the current sector's tables
+                skewAlloc();
+                check(ske_get_tables(e_, target, weight, first, last));
+            }
             check(ske_get_skew(e_, skewInput, skewOutput));
             kernelcpu(this);
             check(ske_set_skew_output(e_, skewOutput));
@@ -106,6 +127,19 @@ class skewEngine {
     // deskew(isMax) (:755) and reduce(output, isMax) (:861).
     void deskew(int isMax = 0) { check(ske_deskew(e_, isMax)); }
     void reduce(T *output, int isMax = 0) { check(ske_reduce(e_, output, isMax)); }
+    // reduce(T **output) (:843-854): sum mode into dimy row pointers of dimx
+    // values; each cell gets o0, o1, o2, o3 added in that order, as in the
+    // flat form.
+    void reduce(T **output) {
+        T *flat = (T *)malloc(sizeof(T) * (size_t)N);
+        if (!flat) throw std::runtime_error("skewEngine: out of host memory");
+        for (int i = 0; i < dimy; i++) memcpy(flat + (size_t)dimx * i, output[i], sizeof(T) * dimx);
+        const int rc = ske_reduce(e_, flat, 0);
+        if (rc == SKE_OK)
+            for (int i = 0; i < dimy; i++) memcpy(output[i], flat + (size_t)dimx * i, sizeof(T) * dimx);
+        free(flat);
+        check(rc);
+    }
 
     // Static helpers of the reference (:378-411, :924-987), on host memory.
     // prepare builds the three mirrored/transposed copies: in1[dimy*cj+ci],
@@ -177,6 +211,21 @@ class skewEngine {
 
   private:
     ske_engine *e_ = nullptr;
+    int planned_ = -1;     // angle of the last computeParams/skew
+    bool skewed_ = false;  // the device holds a skewed sector
+
+    void set_fields(const ske_sector_params &p) {
+        a = p.a;
+        newAngle = p.newAngle;
+        sectorType = p.sectorType;
+        dim_i = p.dim_i;
+        dim_o = p.dim_o;
+        offset = p.offset;
+        skewHeight = p.skewHeight;
+        skewWidth = p.skewWidth;
+        scale = p.scale;
+        skewness = p.skewness;
+    }
 
     static void check(int rc) {
         if (rc != SKE_OK) throw std::runtime_error(std::string("skewEngine: ") + ske_last_error());

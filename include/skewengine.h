@@ -103,6 +103,16 @@ int ske_reduce(ske_engine *e, float *output, int is_max);
 int ske_get_params(ske_engine *e, ske_sector_params *p);
 /* target/weight: dim_i entries; first/last: skewHeight entries. */
 int ske_get_tables(ske_engine *e, int *target, float *weight, unsigned short *first, unsigned short *last);
+/* computeParams(angle) + skewShape() (src/skewEngine.cuh:503-539, 572-596)
+ * without an engine or a device: the public fields and the index maps of
+ * one sector of a dimx x dimy grid (same sizes as ske_get_tables; any
+ * pointer may be NULL).  Host only. */
+int ske_plan_sector(int dimx, int dimy, int angle, ske_sector_params *p, int *target, float *weight,
+                    unsigned short *first, unsigned short *last);
+/* resetSkew() (src/skewEngine.cuh:541-570): zero skewInput and skewOutput
+ * over skewHeight*dim_i cells of the current sector (ske_skew does this
+ * itself before skewing). */
+int ske_reset_skew(ske_engine *e);
 /* skewInput / skewOutput rows (skewHeight*dim_i floats each; either may be NULL). */
 int ske_get_skew(ske_engine *e, float *skewInput, float *skewOutput);
 /* Host-side operators (the kernelcpu form, src/skewEngine.cuh:370,696): after
@@ -131,7 +141,9 @@ int ske_synchronize(ske_engine *e);
 #define SKE_OPT_PROFILE 3       /* 1: time every launch with HIP events on the engine stream */
 #define SKE_OPT_GRAPH 4         /* 1: capture the all-sector run once in a hipGraph and replay it */
 #define SKE_OPT_SDEM_WAVES 5    /* viewshed: waves per workgroup (2, 4, 8); unset: 4, or 2 for rows of
-                                 * fewer than four chunk groups (or chunks) per scan direction */
+                                 * fewer than four chunk groups (or chunks) per scan direction.
+                                 * ske_get_option reports the set value (4 while unset), not
+                                 * the per-chunk automatic choice */
 #define SKE_OPT_SDEM_PERSISTENT 6 /* viewshed: 0 one workgroup per row, k>0 persistent grid of k per CU */
 #define SKE_OPT_SDEM_OCCUPANCY 7  /* viewshed persistent grid: 0 default, 8 compiled for 8 waves/SIMD */
 #define SKE_OPT_RADON_LOCAL 8     /* 1: SKE_KERNEL_RADON computes the local transform (pointRadon per

@@ -58,6 +58,8 @@ def lib():
         L.orc_blur_parameter2.argtypes = [f64p, ctypes.c_int, i32p]
         L.orc_line_cepstrum.restype = ctypes.c_float
         L.orc_line_cepstrum.argtypes = [ctypes.c_int, f32p, ctypes.c_int, ctypes.c_int]
+        L.orc_line_cepstrum_ex.restype = ctypes.c_float
+        L.orc_line_cepstrum_ex.argtypes = [ctypes.c_int, f32p, ctypes.c_int, ctypes.c_int, f64p, f64p]
         L.orc_point_radon.restype = ctypes.c_double
         L.orc_point_radon.argtypes = [P(Params), u16p, u16p, f32p, ctypes.c_int, ctypes.c_int, f64p]
         L.orc_sdem_observer.restype = ctypes.c_float
@@ -157,6 +159,17 @@ def blur_parameter2(v):
 def line_cepstrum(j, row, first, last):
     row = np.ascontiguousarray(row, np.float32)
     return lib().orc_line_cepstrum(j, _p(row, ctypes.c_float), first, last)
+
+
+def line_cepstrum_ex(j, row, first, last):
+    """lineCepstrum (src/kernelBLUR.h:153-228) with its intermediates:
+    (value, the 32 log powers ln|X_k|^2, the 16 peak-rule samples)."""
+    row = np.ascontiguousarray(row, np.float32)
+    logps = np.zeros(32, np.float64)
+    samples = np.zeros(16, np.float64)
+    v = lib().orc_line_cepstrum_ex(j, _p(row, ctypes.c_float), first, last, _p(logps, ctypes.c_double),
+                                   _p(samples, ctypes.c_double))
+    return v, logps, samples
 
 
 def sdem_observer(row, first, last, j, povh):

@@ -352,7 +352,7 @@ double orc_blur_parameter2(const double *v, int n, int *imax_out)
  * 64-tap window (zero outside [first,last)), FFT/8, log power of the first
  * 32 bins, FFT/sqrt(32), v_k = -min(0, Re C_k) for k < 16, peak rule,
  * times 10^4. */
-float orc_line_cepstrum(int j, const float *row, int first, int last)
+float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *samples_out)
 {
     double re[ORC_WIN], im[ORC_WIN];
     int half = ORC_WIN / 2;
@@ -367,15 +367,22 @@ float orc_line_cepstrum(int j, const float *row, int first, int last)
         double PS = re[k] * re[k] + im[k] * im[k];
         re2[k] = log(PS);
         im2[k] = 0.0;
+        if (logps_out) logps_out[k] = re2[k];
     }
     orc_fft(re2, im2, ORC_WIN / 2, 1);
     double samples[ORC_WIN / 4];
     for (int k = 0; k < ORC_WIN / 4; k++) {
         double x = re2[k];
         samples[k] = -((x < 0.0) ? x : 0.0); /* -std::min(0.0, x) */
+        if (samples_out) samples_out[k] = samples[k];
     }
     double val = orc_blur_parameter2(samples, ORC_WIN / 4, NULL);
     return (float)(val * 10000);
+}
+
+float orc_line_cepstrum(int j, const float *row, int first, int last)
+{
+    return orc_line_cepstrum_ex(j, row, first, last, NULL, NULL);
 }
 
 /* kernelRADN.h:114-182 (pointRadon, local Radon transform, LRT=true) for

@@ -8,8 +8,8 @@
  * Every function restates one piece of the reference and cites it as
  * file:line relative to the reference repository root.
  *
- * Parity pinning (what checks each function beyond re-typing it) is listed in
- * oracle/README.md and enforced by tests/test_oracle_*.py.
+ * Parity pinning (what checks each function beyond re-typing it) is enforced
+ * by tests/test_oracle_*.py.
  */
 #ifndef SKE_ORACLE_H
 #define SKE_ORACLE_H
@@ -66,6 +66,9 @@ double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_
                        int i, int j, double *spec_out);
 double orc_blur_parameter2(const double *v, int n, int *imax);
 float orc_line_cepstrum(int j, const float *row, int first, int last);
+/* The same, also returning the intermediates: the 32 log powers ln|X_k|^2
+ * and the 16 samples -min(0, Re C_k) the peak rule scans (either may be NULL). */
+float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *samples_out);
 
 /* One observer's two line-of-sight scans on a staged row; writes the
  * visibility decision of every step (1 = above the horizon) and returns cv. */

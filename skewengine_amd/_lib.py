@@ -25,7 +25,8 @@ MAX_SECTOR = 180
 EXPORTS = [
     "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_create",
     "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
-    "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_get_skew", "ske_set_skew_output",
+    "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_plan_sector",
+    "ske_reset_skew", "ske_get_skew", "ske_set_skew_output",
     "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option", "ske_get_option",
     "ske_profile_get", "ske_counter_get", "ske_profile_reset", "ske_work",
     "ske_batch_create", "ske_batch_create_device", "ske_batch_destroy", "ske_batch_run", "ske_batch_output",
@@ -91,6 +92,8 @@ def lib():
     L.ske_reduce.argtypes = [vp, vp, ctypes.c_int]
     L.ske_get_params.argtypes = [vp, P(SectorParams)]
     L.ske_get_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.ske_plan_sector.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(SectorParams), vp, vp, vp, vp]
+    L.ske_reset_skew.argtypes = [vp]
     L.ske_get_skew.argtypes = [vp, vp, vp]
     L.ske_set_skew_output.argtypes = [vp, vp]
     L.ske_device_pointers.argtypes = [vp, P(vp), P(vp), P(vp), P(vp)]

@@ -106,9 +106,11 @@ int kernel_from_name(const char* s) {
     return -1;
 }
 
-}  // namespace
+// Largest grid side the engine accepts (first/last are 16-bit, one row plus
+// guard bands must fit the LDS of a workgroup).
+constexpr int kMaxSide = 16000;
 
-extern "C" int run(int argc, char* argv[]) {
+int run_impl(int argc, char* argv[]) {
     int kernel = SKE_KERNEL_SDEM;
     int local = 0;
     std::string input, output, png_dir;
@@ -152,6 +154,14 @@ extern "C" int run(int argc, char* argv[]) {
         fprintf(stderr, "run: unknown --kernel\n");
         return SKE_EINVAL;
     }
+    // --dims crops a square corner of the input (readBilData reads the first
+    // dimx columns of the first dimy rows, src/kernelSDEM.h:43-50); it can
+    // never enlarge the grid.
+    const bool dims_given = dimx != 0 || dimy != 0;
+    if (dims_given && (dimx != dimy || dimx < 2 || dimx > kMaxSide)) {
+        fprintf(stderr, "run: --dims must be NxN with 2 <= N <= %d\n", kMaxSide);
+        return SKE_EINVAL;
+    }
     std::vector<float> grid;
     if (input.empty()) {
         dimx = dimy = 500;
@@ -173,7 +183,11 @@ extern "C" int run(int argc, char* argv[]) {
             return SKE_EINVAL;
         }
         const int side = (int)sqrt((double)(sz / 2));
-        if (dimx <= 0) dimx = dimy = side;
+        if (!dims_given) dimx = dimy = side;
+        if (side < 2 || side > kMaxSide || dimx > side) {
+            fprintf(stderr, "run: %s holds a %dx%d grid; --dims cannot exceed it\n", input.c_str(), side, side);
+            return SKE_EINVAL;
+        }
         if (!read_bil(input.c_str(), side, dimx, dimy, grid)) {
             fprintf(stderr, "run: short read on %s\n", input.c_str());
             return SKE_EINVAL;
@@ -184,7 +198,12 @@ extern "C" int run(int argc, char* argv[]) {
             fprintf(stderr, "run: cannot open %s\n", input.c_str());
             return SKE_EINVAL;
         }
-        if (dimx <= 0) dimx = dimy = (int)sqrt((double)(sz / 4));
+        const int side = (int)sqrt((double)(sz / 4));
+        if (!dims_given) dimx = dimy = side;
+        if (side < 2 || (long)dimx * dimy * 4 > sz || dimx > kMaxSide) {
+            fprintf(stderr, "run: %s holds fewer than %dx%d floats\n", input.c_str(), dimx, dimy);
+            return SKE_EINVAL;
+        }
         if (!read_flt(input.c_str(), dimx, dimy, grid)) {
             fprintf(stderr, "run: short read on %s\n", input.c_str());
             return SKE_EINVAL;
@@ -228,4 +247,15 @@ extern "C" int run(int argc, char* argv[]) {
         }
     }
     return SKE_OK;
+}
+
+}  // namespace
+
+extern "C" int run(int argc, char* argv[]) {
+    try {
+        return run_impl(argc, argv);
+    } catch (...) {  // allocation failures must not cross the C boundary
+        fprintf(stderr, "run: out of host memory\n");
+        return SKE_ENOMEM;
+    }
 }

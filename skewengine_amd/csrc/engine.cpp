@@ -470,9 +470,11 @@ struct Core {
         char key[160];
         unsigned pbits = 0;
         memcpy(&pbits, &povh, sizeof pbits);
-        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%d:%lld:%d:%d", kernel_id, s0, s1, pbits, sdem_variant,
-                 tune.waves, tune.persistent_wgs_per_cu, tune.occupancy, tune.group, arena_bytes,
-                 radon_local ? 1 : 0, count_steps ? 1 : 0);
+        // every setting that changes what enqueue_run records is part of the key
+        // (waves_set: unset wave counts are chosen per chunk by effective_tuning)
+        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%d:%d:%lld:%d:%d", kernel_id, s0, s1, pbits,
+                 sdem_variant, tune.waves, waves_set ? 1 : 0, tune.persistent_wgs_per_cu, tune.occupancy, tune.group,
+                 arena_bytes, radon_local ? 1 : 0, count_steps ? 1 : 0);
         if (!gexec || gkey != key) {
             drop_graph();
             SKE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
@@ -970,6 +972,42 @@ int ske_work(int dimx, int dimy, int s0, int s1, double* cells, double* steps) {
     if (cells) *cells = ip.cells;
     if (steps) *steps = ip.steps;
     return SKE_OK;
+}
+
+int ske_plan_sector(int dimx, int dimy, int angle, ske_sector_params* p, int* target, float* weight,
+                    unsigned short* first, unsigned short* last) {
+    SKE_TRY(check_dims(dimx, dimy));
+    if (angle < 0 || angle >= SKE_MAX_SECTOR) return fail(SKE_EINVAL, "angle must be in [0,180)");
+    SectorPlan sp;
+    plan_sector(dimx, dimy, angle, sp);
+    if (p) {
+        p->a = sp.angle;
+        p->newAngle = sp.newAngle;
+        p->sectorType = sp.type;
+        p->dim_i = sp.dim_i;
+        p->dim_o = sp.dim_o;
+        p->offset = sp.offset;
+        p->skewHeight = sp.H;
+        p->skewWidth = sp.skewWidth;
+        p->scale = sp.scale;
+        p->skewness = sp.skewness;
+    }
+    if (target) memcpy(target, sp.target.data(), sizeof(int) * sp.dim_i);
+    if (weight) memcpy(weight, sp.weight.data(), sizeof(float) * sp.dim_i);
+    if (first) memcpy(first, sp.first.data(), sizeof(uint16_t) * sp.H);
+    if (last) memcpy(last, sp.last.data(), sizeof(uint16_t) * sp.H);
+    return SKE_OK;
+}
+
+int ske_reset_skew(ske_engine* e) {
+    if (!e) return fail(SKE_EINVAL, "engine is NULL");
+    if (!e->have_sector) return fail(SKE_ESTATE, "no sector skewed yet");
+    Core& c = e->core;
+    SKE_HIP(hipSetDevice(c.device));
+    const size_t cells = (size_t)e->cur.H * e->cur.dim_i;
+    SKE_HIP(hipMemsetAsync(e->S.p, 0, cells * sizeof(float), c.stream));
+    SKE_HIP(hipMemsetAsync(e->SO.p, 0, cells * sizeof(float), c.stream));
+    return c.sync();
 }
 
 int ske_execute(int kernel_id, int dimx, int dimy, const float* input, float* output, float povh, int sector_begin,

@@ -247,6 +247,21 @@ def work(dimx, dimy, s0=0, s1=180):
     return c.value, s.value
 
 
+def plan_sector(dimx, dimy, angle):
+    """computeParams(angle) + skewShape() (src/skewEngine.cuh:503-539,
+    572-596) on the host: (params, target, weight, first, last) with
+    target/weight of dim_i entries and first/last of skewHeight entries."""
+    p = SectorParams()
+    check(lib().ske_plan_sector(dimx, dimy, angle, ctypes.byref(p), None, None, None, None))
+    target = np.zeros(p.dim_i, np.int32)
+    weight = np.zeros(p.dim_i, np.float32)
+    first = np.zeros(p.skewHeight, np.uint16)
+    last = np.zeros(p.skewHeight, np.uint16)
+    check(lib().ske_plan_sector(dimx, dimy, angle, ctypes.byref(p), target.ctypes.data, weight.ctypes.data,
+                                first.ctypes.data, last.ctypes.data))
+    return p, target, weight, first, last
+
+
 def device_count():
     n = ctypes.c_int(0)
     check(lib().ske_device_count(ctypes.byref(n)))

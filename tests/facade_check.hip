@@ -29,8 +29,12 @@ static void identity_host(skewEngine<float>* e) {
 }
 
 // mode 0: built-in operator `op`; 1: kernelcpu identity; 2: kernelcuda
-// identity.  Sectors [s0, s1), deskew/reduce mode is_max.  Returns 0 or -1.
-extern "C" int facade_farm(int mode, int op, int n, const float* grid, int s0, int s1, int is_max, float* out) {
+// identity.  Sectors [s0, s1), deskew/reduce mode is_max.  flags: bit 0
+// leaves skewAlloc() to kernel(); bit 1 reduces through row pointers
+// (reduce(T**), sum mode); bit 2 calls resetSkew() after the last sector's
+// kernel(), so that sector adds nothing.  Returns 0 or -1.
+extern "C" int facade_farm(int mode, int op, int n, const float* grid, int s0, int s1, int is_max, int flags,
+                           float* out) {
     try {
         inputData<float> in;
         float* outD = nullptr;
@@ -39,16 +43,24 @@ extern "C" int facade_farm(int mode, int op, int n, const float* grid, int s0, i
         skewEngine<float>::allocate(in, inputD, &outD, n, n);
         {
             skewEngine<float> e(n, n, in, true, 0);
-            e.skewAlloc();
+            if (!(flags & 1)) e.skewAlloc();
             if (mode == 0) e.operatorId = op;
             if (mode == 1) e.kernelcpu = identity_host;
             if (mode == 2) e.kernelcuda = identity_device;
             for (int a = s0; a < s1; a++) {
                 e.skew(a);
                 e.kernel();
+                if ((flags & 4) && a == s1 - 1) e.resetSkew();
                 e.deskew(is_max);
             }
-            e.reduce(outD, is_max);
+            if (flags & 2) {
+                float** rows = (float**)malloc(sizeof(float*) * n);
+                for (int i = 0; i < n; i++) rows[i] = outD + (size_t)n * i;
+                e.reduce(rows);
+                free(rows);
+            } else {
+                e.reduce(outD, is_max);
+            }
         }
         memcpy(out, outD, sizeof(float) * (size_t)n * n);
         skewEngine<float>::deallocate(in, inputD, outD);
@@ -81,6 +93,30 @@ extern "C" int facade_fields(int n, const float* grid, int angle, int* ints, uns
         skewEngine<float> e(n, n, in, true, 0);
         e.skewAlloc();
         e.skew(angle);
+        const int v[8] = {e.a, e.newAngle, e.sectorType, e.dim_i, e.dim_o, e.offset, e.skewHeight, e.skewWidth};
+        memcpy(ints, v, sizeof v);
+        memcpy(first, e.first, sizeof(unsigned short) * e.skewHeight);
+        memcpy(last, e.last, sizeof(unsigned short) * e.skewHeight);
+        memcpy(target, e.target, sizeof(int) * e.dim_i);
+        memcpy(weight, e.weight, sizeof(float) * e.dim_i);
+        scale[0] = e.scale;
+        scale[1] = e.skewness;
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// computeParams(angle) + skewShape() without skew(): the same fields and
+// maps as facade_fields, from the host planner alone.
+extern "C" int facade_plan(int n, const float* grid, int angle, int* ints, unsigned short* first,
+                           unsigned short* last, int* target, float* weight, double* scale) {
+    try {
+        inputData<float> in;
+        in.input0 = const_cast<float*>(grid);
+        skewEngine<float> e(n, n, in, true, 0);
+        e.computeParams(angle);
+        e.skewShape();
         const int v[8] = {e.a, e.newAngle, e.sectorType, e.dim_i, e.dim_o, e.offset, e.skewHeight, e.skewWidth};
         memcpy(ints, v, sizeof v);
         memcpy(first, e.first, sizeof(unsigned short) * e.skewHeight);

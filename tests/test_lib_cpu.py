@@ -57,3 +57,39 @@ def test_argument_errors_without_device():
     assert L.ske_work(20000, 20000, 0, 180, None, None) == -1
     with pytest.raises(ske.SkeError):
         ske.work(8, 8, 5, 200)
+
+
+def test_run_rejects_dims_beyond_the_input(tmp_path):
+    """--dims may only crop a square corner of the file (readBilData,
+    src/kernelSDEM.h:43-50): larger, non-square or negative sizes are refused
+    before any device work."""
+    bil = tmp_path / "dem.bil"
+    np.zeros((16, 16), np.int16).tofile(bil)
+    flt = tmp_path / "dem.flt"
+    np.zeros((16, 16), np.float32).tofile(flt)
+    for f in (bil, flt):
+        for dims in ("17x17", "32x32", "8x12", "-4x-4", "1x1", "16001x16001"):
+            assert ske.run_cli(["--input", str(f), "--dims", dims]) == _lib.SKE_OK - 1, (f, dims)
+    # a BIL file too small to hold a 2x2 grid
+    tiny = tmp_path / "tiny.bil"
+    np.zeros(3, np.int16).tofile(tiny)
+    assert ske.run_cli(["--input", str(tiny)]) == _lib.SKE_OK - 1
+
+
+@pytest.mark.parametrize("n", [5, 64, 500])
+def test_plan_sector_matches_oracle_maps(n):
+    """ske_plan_sector (computeParams + skewShape on the host, through the C
+    ABI) equals the oracle's maps bit for bit on every sector."""
+    for a in range(180):
+        p, t, w, f, l = ske.plan_sector(n, n, a)
+        rp, rt, rw, rf, rl = oracle.compute_params(n, n, a)
+        assert (p.a, p.newAngle, p.sectorType, p.dim_i, p.dim_o, p.offset, p.skewHeight, p.skewWidth) == (
+            rp.a, rp.newAngle, rp.sectorType, rp.dim_i, rp.dim_o, rp.offset, rp.skewHeight, rp.skewWidth)
+        assert p.scale == rp.scale and p.skewness == rp.skewness
+        H = p.skewHeight
+        assert np.array_equal(t, rt[:n]) and np.array_equal(w, rw[:n])
+        assert np.array_equal(f, rf[:H]) and np.array_equal(l, rl[:H]), a
+    with pytest.raises(ske.SkeError):
+        ske.plan_sector(n, n, 180)
+    with pytest.raises(ske.SkeError):
+        ske.plan_sector(n, n + 1, 0)
