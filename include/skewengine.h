@@ -58,9 +58,12 @@ int ske_device_count(int *count);
 
 /* DLL entry point: extern "C" int run(int argc, char *argv[])
  * (src/main.cu:199-202).  argv: [--kernel ID] [--input FILE(.flt|.bil|.png)]
- * [--dims WxH] [--output FILE.flt] [--sectors A:B] [--povh H] [--device D].
- * With no arguments it runs the total viewshed on a built-in 500x500
- * synthetic DEM.  Returns 0 on success. */
+ * [--dims NxN (crop)] [--output FILE.flt] [--sectors A:B] [--povh H]
+ * [--device D] [--local] [--png-dir DIR] [--sample-dir DIR [--samples
+ * x,y;x,y;...]] (--sample-dir: the reference's saveSampleData dump for the
+ * Radon and identity-image operators; default points those of
+ * src/global.cpp:54-56).  With no arguments it runs the total viewshed on a
+ * built-in 500x500 synthetic DEM.  Returns 0 on success. */
 int run(int argc, char **argv);
 
 /* Whole-image path: allocate + prepare + the sector farm + reduce
@@ -103,6 +106,26 @@ int ske_reduce(ske_engine *e, float *output, int is_max);
 int ske_get_params(ske_engine *e, ske_sector_params *p);
 /* target/weight: dim_i entries; first/last: skewHeight entries. */
 int ske_get_tables(ske_engine *e, int *target, float *weight, unsigned short *first, unsigned short *last);
+/* Sample points (saveSampleData and samplePoints, src/global.cpp:47-56,
+ * src/main.cu:79-81): record, for `count` image cells (xy: x0, y0, x1, y1,
+ * ... of grid 0; at most 16) and every sector, the four 64-float vectors the
+ * local Radon transform (SKE_OPT_RADON_LOCAL) forms at that cell
+ * (pointRadon, src/kernelRADN.h:164-170): the Gaussian-weighted window sums,
+ * the weighted row through the cell, the filtered spectrum and the real part
+ * of its transform.  count 0 stops recording.  Setting points clears the
+ * record; each run that records overwrites it. */
+int ske_set_sample_points(ske_engine *e, int count, const int *xy);
+/* The record: 180 x count x 4 x 64 floats (sector, point, vector, index).
+ * Zeros where nothing was recorded: sectors not run, operators other than
+ * the local transform, cells outside their skewed row (as in the reference,
+ * which visits only cells inside [first, last)). */
+int ske_get_sample_data(ske_engine *e, float *out);
+/* The sample dump of showResultsRADN (src/kernelRADN.h:263-382): files
+ * punto<p>o.txt, punto<p>i.txt, punto<p>l.txt, punto<p>c.txt (p = 1..count)
+ * in directory dir from a record laid out as ske_get_sample_data returns
+ * it, in the reference's text layout (two blocks of 180 lines of 32 values). */
+int ske_write_sample_files(const char *dir, int count, const float *data);
+
 /* computeParams(angle) + skewShape() (src/skewEngine.cuh:503-539, 572-596)
  * without an engine or a device: the public fields and the index maps of
  * one sector of a dimx x dimy grid (same sizes as ske_get_tables; any
@@ -255,24 +278,29 @@ int ske_result_pngs(const char *dir, int kernel_id, int width, int height, const
  *   r0  = BP(h * p), h[m] = taps[|m|] (n_det symmetric taps; the DCT of the
  *         taps is the filter's per-frequency response)
  *   out = r0 + conv(relu(conv(relu(conv(relu(conv(r0))))))) with 3x3 convs
- *         1->16->16->16->1, zero padding, bf16 weights and activations,
- *         fp32 accumulation.
- * Parameters (fp32, SKE_LEARNED_TAPS + n_det of them): W1[16][9] b1[16]
- * W2[16][9][16] b2[16] W3[16][9][16] b3[16] W4[9][16] b4 taps[n_det]; tap
- * t = 3*ky + kx reads pixel (i+ky-1, j+kx-1).  The image side must be a
- * multiple of 32. */
+ *         1->C->C->C->1 (C = 16 or 32 hidden channels), zero padding, bf16
+ *         weights and activations, fp32 accumulation.
+ * Parameters (fp32, taps offset + n_det of them): W1[C][9] b1[C]
+ * W2[C][9][C] b2[C] W3[C][9][C] b3[C] W4[9][C] b4 taps[n_det]; tap
+ * t = 3*ky + kx reads pixel (i+ky-1, j+kx-1).  The taps offset is
+ * SKE_LEARNED_TAPS for C = 16 and SKE_LEARNED_TAPS_C32 for C = 32.  The
+ * image side must be a multiple of 32. */
 typedef struct ske_learned ske_learned;
-#define SKE_LEARNED_TAPS 4945  /* offset of the filter taps in the parameters */
+#define SKE_LEARNED_TAPS 4945  /* offset of the filter taps in the parameters (16 channels) */
+#define SKE_LEARNED_TAPS_C32 19105  /* the same for 32 hidden channels */
 #define SKE_LEARNED_R0 0       /* fp32 batch x n x n: learned-filter FBP */
-#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x 16: hidden activations of the last
+#define SKE_LEARNED_A1 1       /* bf16 (uint16) batch x n x n x C: hidden activations of the last
                                 * ske_learned_gradient / train_step (reconstruct keeps them in LDS) */
 #define SKE_LEARNED_A2 2
 #define SKE_LEARNED_A3 3
 #define SKE_LEARNED_DR0 4      /* fp32 batch x n x n: d loss / d r0 (after a gradient) */
 #define SKE_LEARNED_FILTERED_GRAD 5 /* fp32 batch x n_angles x n_det: d loss / d filtered sinogram */
-int ske_learned_create(ske_fbp *f, ske_learned **out);
+int ske_learned_create(ske_fbp *f, ske_learned **out); /* 16 hidden channels */
+int ske_learned_create_channels(ske_fbp *f, int channels, ske_learned **out); /* channels: 16 or 32 */
 int ske_learned_destroy(ske_learned *l);
 int ske_learned_param_count(ske_learned *l, int *count);
+/* hidden channels and the offset of the filter taps in the parameters */
+int ske_learned_layout(ske_learned *l, int *channels, int *taps_offset);
 /* He-normal hidden weights from seed, zero biases, zero output layer and
  * Ram-Lak taps: the output equals plain FBP until trained. */
 int ske_learned_init(ske_learned *l, unsigned long long seed);

@@ -6,17 +6,19 @@ CNN).  This is a float64 numpy restatement of the model the GPU kernels
 compute, forward and backward:
 
   r0  = BP(h * p),  h[m] = taps[|m|]                       (std_fbp geometry)
-  a1  = bf(relu(conv(bf(r0); bf(W1)) + b1))               1 -> 16
-  a2  = bf(relu(conv(a1; bf(W2)) + b2))                   16 -> 16
-  a3  = bf(relu(conv(a2; bf(W3)) + b3))                   16 -> 16
-  out = r0 + conv(a3; bf(W4)) + b4                        16 -> 1
+  a1  = bf(relu(conv(bf(r0); bf(W1)) + b1))               1 -> C
+  a2  = bf(relu(conv(a1; bf(W2)) + b2))                   C -> C
+  a3  = bf(relu(conv(a2; bf(W3)) + b3))                   C -> C
+  out = r0 + conv(a3; bf(W4)) + b4                        C -> 1
+
+with C = 16 or 32 hidden channels (keyword C, default 16).
   loss = mean((out - target)^2)
 
 bf() rounds to bfloat16 (nearest even) when `bf16=True`, the identity
 otherwise; gradients pass straight through bf() (the GPU differentiates the
 same way).  3x3 convolutions with zero padding, tap t = 3*ky + kx reading
-pixel (i+ky-1, j+kx-1).  Parameter vector layout: W1[16][9] b1[16]
-W2[16][9][16] b2[16] W3[16][9][16] b3[16] W4[9][16] b4 taps[n_det].
+pixel (i+ky-1, j+kx-1).  Parameter vector layout: W1[C][9] b1[C]
+W2[C][9][C] b2[C] W3[C][9][C] b3[C] W4[9][C] b4 taps[n_det].
 
 Pinned by tests/test_oracle_learned.py: central finite differences of the
 FP64 loss (bf16 off) for every parameter block and for r0, the adjoint
@@ -27,14 +29,20 @@ import numpy as np
 
 from . import std_fbp
 
+def layout(C):
+    """Parameter layout for C hidden channels: ({block: (offset, size)},
+    offset of the taps)."""
+    off = {}
+    o = 0
+    for name, size in (("W1", C * 9), ("b1", C), ("W2", C * 9 * C), ("b2", C), ("W3", C * 9 * C), ("b3", C),
+                       ("W4", 9 * C), ("b4", 1)):
+        off[name] = (o, size)
+        o += size
+    return off, o
+
+
 C = 16
-OFF = {}
-_o = 0
-for _name, _size in (("W1", C * 9), ("b1", C), ("W2", C * 9 * C), ("b2", C), ("W3", C * 9 * C), ("b3", C),
-                     ("W4", 9 * C), ("b4", 1)):
-    OFF[_name] = (_o, _size)
-    _o += _size
-TAPS = _o  # 4945
+OFF, TAPS = layout(C)  # the 16-channel layout (TAPS = 4945)
 
 
 def bf16(x):
@@ -45,24 +53,26 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def unpack(prm):
+def unpack(prm, C=16):
     prm = np.asarray(prm, np.float64)
+    off, taps = layout(C)
     g = {}
-    for k, (o, s) in OFF.items():
+    for k, (o, s) in off.items():
         g[k] = prm[o:o + s]
     g["W1"] = g["W1"].reshape(C, 9)
     g["W2"] = g["W2"].reshape(C, 9, C)
     g["W3"] = g["W3"].reshape(C, 9, C)
     g["W4"] = g["W4"].reshape(9, C)
-    g["taps"] = prm[TAPS:]
+    g["taps"] = prm[taps:]
     return g
 
 
-def pack(g, n_det):
-    out = np.zeros(TAPS + n_det)
-    for k, (o, s) in OFF.items():
+def pack(g, n_det, C=16):
+    off, taps = layout(C)
+    out = np.zeros(taps + n_det)
+    for k, (o, s) in off.items():
         out[o:o + s] = np.asarray(g[k]).reshape(-1)
-    out[TAPS:] = g["taps"]
+    out[taps:] = g["taps"]
     return out
 
 
@@ -103,9 +113,9 @@ def conv3x3_weight_grad(dz, x):
     return dW
 
 
-def cnn_forward(prm, r0, bf=True):
+def cnn_forward(prm, r0, bf=True, C=16):
     """r0 (B, n, n) -> dict with the activations and `out`."""
-    g = unpack(prm)
+    g = unpack(prm, C)
     q = bf16 if bf else (lambda v: np.asarray(v, np.float64))
     r0 = np.asarray(r0, np.float64)
     x0 = q(r0)[..., None]
@@ -119,10 +129,10 @@ def cnn_forward(prm, r0, bf=True):
     return {"r0": r0, "x0": x0, "a1": a1, "a2": a2, "a3": a3, "out": out}
 
 
-def cnn_backward(prm, cache, dout, bf=True):
+def cnn_backward(prm, cache, dout, bf=True, C=16):
     """Gradients of a scalar loss with d loss/d out = dout (B, n, n): the
     parameter gradient (taps part zero) and d loss / d r0."""
-    g = unpack(prm)
+    g = unpack(prm, C)
     q = bf16 if bf else (lambda v: np.asarray(v, np.float64))
     W1, W2, W3, W4 = q(g["W1"])[:, :, None], q(g["W2"]), q(g["W3"]), q(g["W4"])[None, :, :]
     d = dout[..., None]
@@ -189,35 +199,35 @@ def fold_taps(dfull):
     return out
 
 
-def learned_fbp(prm, sino, n, angles=None):
-    """sino (B, nang, ndet) -> (r0, cache) with r0 the learned-filter FBP."""
-    g = unpack(prm)
+def learned_fbp(prm, sino, n, angles=None, C=16):
+    """sino (B, nang, ndet) -> r0, the learned-filter FBP."""
+    g = unpack(prm, C)
     h = full_taps(g["taps"])
     sino = np.asarray(sino, np.float64)
     r0 = np.stack([std_fbp.backproject(std_fbp.filter_rows(s, h), n, angles) for s in sino])
     return r0
 
 
-def loss_and_grad(prm, sino, target, bf=True, angles=None):
+def loss_and_grad(prm, sino, target, bf=True, angles=None, C=16):
     """Full model: loss = mean((out - target)^2) and its gradient w.r.t. every
     parameter (taps included)."""
     sino = np.asarray(sino, np.float64)
     target = np.asarray(target, np.float64)
     B, n_angles, n_det = sino.shape
     n = target.shape[-1]
-    r0 = learned_fbp(prm, sino, n, angles)
-    cache = cnn_forward(prm, r0, bf)
+    r0 = learned_fbp(prm, sino, n, angles, C)
+    cache = cnn_forward(prm, r0, bf, C)
     diff = cache["out"] - target
     count = diff.size
     loss = float((diff ** 2).sum() / count)
     dout = 2.0 * diff / count
-    grads, dr0 = cnn_backward(prm, cache, dout, bf)
+    grads, dr0 = cnn_backward(prm, cache, dout, bf, C)
     dfull = np.zeros(2 * n_det - 1)
     for b in range(B):
         dq = backproject_adjoint(dr0[b], n_angles, n_det, angles)
         dfull += filter_grad(dq, sino[b])
     grads["taps"] = fold_taps(dfull)
-    return loss, pack(grads, n_det), {"cache": cache, "dr0": dr0, "dout": dout}
+    return loss, pack(grads, n_det, C), {"cache": cache, "dr0": dr0, "dout": dout}
 
 
 def psnr(rec, target):

@@ -62,12 +62,17 @@ def lib():
         L.orc_line_cepstrum_ex.argtypes = [ctypes.c_int, f32p, ctypes.c_int, ctypes.c_int, f64p, f64p]
         L.orc_point_radon.restype = ctypes.c_double
         L.orc_point_radon.argtypes = [P(Params), u16p, u16p, f32p, ctypes.c_int, ctypes.c_int, f64p]
+        L.orc_point_radon_samples.restype = ctypes.c_double
+        L.orc_point_radon_samples.argtypes = [P(Params), u16p, u16p, f32p, ctypes.c_int, ctypes.c_int, f32p]
         L.orc_sdem_observer.restype = ctypes.c_float
         L.orc_sdem_observer.argtypes = [f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, u8p, u8p]
         L.orc_default_threads.restype = ctypes.c_int
         L.orc_execute.restype = ctypes.c_int
         L.orc_execute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, f32p, f32p, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_float, f64p]
+        L.orc_execute_sdem_f64.restype = ctypes.c_int
+        L.orc_execute_sdem_f64.argtypes = [ctypes.c_int, ctypes.c_int, f32p, f64p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_double]
         L.orc_sector.restype = ctypes.c_int
         L.orc_sector.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, f32p, ctypes.c_int, ctypes.c_float,
                                  P(Params), i32p, f32p, u16p, u16p, f32p, f32p]
@@ -163,13 +168,14 @@ def line_cepstrum(j, row, first, last):
 
 def line_cepstrum_ex(j, row, first, last):
     """lineCepstrum (src/kernelBLUR.h:153-228) with its intermediates:
-    (value, the 32 log powers ln|X_k|^2, the 16 peak-rule samples)."""
+    (value, the 32 log powers ln|X_k|^2, the 16 cepstrum values Re C_k whose
+    -min(0, .) are the peak rule's samples)."""
     row = np.ascontiguousarray(row, np.float32)
     logps = np.zeros(32, np.float64)
-    samples = np.zeros(16, np.float64)
+    ceps = np.zeros(16, np.float64)
     v = lib().orc_line_cepstrum_ex(j, _p(row, ctypes.c_float), first, last, _p(logps, ctypes.c_double),
-                                   _p(samples, ctypes.c_double))
-    return v, logps, samples
+                                   _p(ceps, ctypes.c_double))
+    return v, logps, ceps
 
 
 def sdem_observer(row, first, last, j, povh):
@@ -192,6 +198,31 @@ def point_radon(sec, i, j):
     return v, spec
 
 
+def point_radon_samples(sec, i, j):
+    """The four vectors pointRadon records for a sample point
+    (src/kernelRADN.h:164-170) at skewed cell (i, j) of a sector() result:
+    float32 array (4, 64) = windowed sums, axis, filtered spectrum, real
+    part of its transform."""
+    out = np.zeros(4 * 64, np.float32)
+    lib().orc_point_radon_samples(ctypes.byref(sec["p"]), _p(sec["first"], ctypes.c_uint16),
+                                  _p(sec["last"], ctypes.c_uint16), _p(sec["S"], ctypes.c_float), i, j,
+                                  _p(out, ctypes.c_float))
+    return out.reshape(4, 64)
+
+
+def skewed_cell(p, target, x, y, dimx, dimy):
+    """The skewed cell (i, j) of sector p whose source is image cell (x, y):
+    getXY (src/kernelRADN.h:28-41) inverted."""
+    tr = p.sectorType in (1, 2)
+    rx = p.sectorType in (1, 3)
+    ry = p.sectorType in (1, 2)
+    xx = dimx - 1 - x if rx else x
+    yy = dimy - 1 - y if ry else y
+    j = yy if tr else xx
+    i = (xx if tr else yy) + int(target[j])
+    return i, j
+
+
 def default_threads():
     return lib().orc_default_threads()
 
@@ -207,6 +238,20 @@ def execute(alg, img, nthreads=0, s0=0, s1=180, povh=0.15):
     if rc != 0:
         raise RuntimeError("oracle execute failed")
     return out.reshape(dimy, dimx), secs.value
+
+
+def execute_sdem_f64(img, nthreads=0, s0=0, s1=180, povh=0.15):
+    """FP64 restatement of the total-viewshed farm (kernelV3,
+    src/kernelSDEM.h:168-222): the same maps and scans in double; povh is
+    taken as the double nearest the given value."""
+    img = np.ascontiguousarray(img, np.float32)
+    dimy, dimx = img.shape
+    out = np.zeros(dimx * dimy, np.float64)
+    rc = lib().orc_execute_sdem_f64(dimx, dimy, _p(img, ctypes.c_float), _p(out, ctypes.c_double), nthreads,
+                                    s0, s1, float(povh))
+    if rc != 0:
+        raise RuntimeError("oracle FP64 execute failed")
+    return out.reshape(dimy, dimx)
 
 
 def sector(alg, img, angle, povh=0.15):

@@ -352,7 +352,7 @@ double orc_blur_parameter2(const double *v, int n, int *imax_out)
  * 64-tap window (zero outside [first,last)), FFT/8, log power of the first
  * 32 bins, FFT/sqrt(32), v_k = -min(0, Re C_k) for k < 16, peak rule,
  * times 10^4. */
-float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *samples_out)
+float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *ceps_out)
 {
     double re[ORC_WIN], im[ORC_WIN];
     int half = ORC_WIN / 2;
@@ -374,7 +374,7 @@ float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double 
     for (int k = 0; k < ORC_WIN / 4; k++) {
         double x = re2[k];
         samples[k] = -((x < 0.0) ? x : 0.0); /* -std::min(0.0, x) */
-        if (samples_out) samples_out[k] = samples[k];
+        if (ceps_out) ceps_out[k] = x;
     }
     double val = orc_blur_parameter2(samples, ORC_WIN / 4, NULL);
     return (float)(val * 10000);
@@ -391,18 +391,25 @@ float orc_line_cepstrum(int j, const float *row, int first, int last)
  * [0, H)) times a Gaussian of sigma = (64/5)/scale, FFT/8, a sine ramp on
  * the real parts of bins 0..25 (zero above 25.6), FFT/8 again, and the
  * squared magnitude at the first rise after bin 5 (getParameter :96-108). */
-double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
-                       int i, int j, double *spec_out)
+/* samples_out (4 x 64 floats, may be NULL): the vectors pointRadon records
+ * for a sample point (kernelRADN.h:164-170): the windowed sums (input1), the
+ * Gaussian-weighted row through the cell (axis, with the extents of row k of
+ * the window as the reference reads them; rows outside [0, H) give 0), the
+ * filtered spectrum (input2) and the real part of its transform (data2). */
+static double point_radon_core(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                               int i, int j, double *spec_out, float *samples_out)
 {
     const int w = p->skewWidth, h = p->skewHeight, w2 = ORC_WIN / 2;
     const double dws = ORC_WIN;
     double sigma = ORC_WIN / 5.0;
     sigma = sigma / p->scale;
-    double re[ORC_WIN], im[ORC_WIN];
+    double re[ORC_WIN], im[ORC_WIN], axis[ORC_WIN];
     for (int k = i - w2; k < i + w2; k++) {
         double accum = 0;
+        int fi = 0, la = 0;
         if (k >= 0 && k < h) {
-            int fi = first[k], la = last[k];
+            fi = first[k];
+            la = last[k];
             for (int l = j - w2; l <= j + w2; l++)
                 accum += (l < fi || l >= la) ? 0 : S[(size_t)k * w + l];
         }
@@ -410,7 +417,14 @@ double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_
         double gauss = exp(-(kk - w2) * (kk - w2) / (2 * sigma * sigma));
         re[kk] = accum * gauss;
         im[kk] = 0.0;
+        int c = j + k - i;
+        axis[kk] = (c < fi) ? 0 : (c >= la ? 0 : gauss * S[(size_t)i * w + c]);
     }
+    if (samples_out)
+        for (int k = 0; k < ORC_WIN; k++) {
+            samples_out[k] = (float)re[k];
+            samples_out[ORC_WIN + k] = (float)axis[k];
+        }
     orc_fft(re, im, ORC_WIN, 1);
     double cutoff = 0.4;
     double re2[ORC_WIN], im2[ORC_WIN];
@@ -418,7 +432,11 @@ double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_
         re2[k] = k > cutoff * dws ? 0 : sin((k / (cutoff * dws - 1)) * (M_PI / 2)) * re[k];
         im2[k] = 0.0;
     }
+    if (samples_out)
+        for (int k = 0; k < ORC_WIN; k++) samples_out[2 * ORC_WIN + k] = (float)re2[k];
     orc_fft(re2, im2, ORC_WIN, 1);
+    if (samples_out)
+        for (int k = 0; k < ORC_WIN; k++) samples_out[3 * ORC_WIN + k] = (float)re2[k];
     if (spec_out)
         for (int k = 0; k < ORC_WIN; k++) spec_out[k] = re2[k] * re2[k] + im2[k] * im2[k];
     int n = w2, ii;
@@ -426,6 +444,18 @@ double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_
         if (re2[ii] * re2[ii] + im2[ii] * im2[ii] > re2[ii - 1] * re2[ii - 1] + im2[ii - 1] * im2[ii - 1]) break;
     if (ii == n) return 0.0;
     return re2[ii - 1] * re2[ii - 1] + im2[ii - 1] * im2[ii - 1];
+}
+
+double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                       int i, int j, double *spec_out)
+{
+    return point_radon_core(p, first, last, S, i, j, spec_out, NULL);
+}
+
+double orc_point_radon_samples(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                               int i, int j, float *samples_out)
+{
+    return point_radon_core(p, first, last, S, i, j, NULL, samples_out);
 }
 
 /* kernelRADN.h:214-220 (radon with LRT=true). */
@@ -553,6 +583,118 @@ int orc_execute(int alg, int dimx, int dimy, const float *input, float *out,
     }
     double t2 = omp_get_wtime();
     if (seconds) *seconds = t2 - t1;
+    free(in1); free(in2); free(in3);
+    return err;
+}
+
+/* FP64 restatement of the total-viewshed farm (kernelV3, kernelSDEM.h:168-222,
+ * driven as main.cu:71-172): the same index maps and scans, with the skew
+ * weights (skewness*j - target[j]), the skewed heights, the elevation angles,
+ * the observer height, cv, the area factor scale^2, the deskew and the reduce
+ * all in double.  Its totals are what the FP32 algorithm approximates: the
+ * gap measures the float rounding of the reference's own arithmetic. */
+static double observer_f64(const double *row, int inicio, int final, int j, double povh)
+{
+    double cv = 0, max_angle, open_delta_d, delta_d, cAngle;
+    int visible, above;
+    double h = row[j] + povh;
+    for (int dir = 1; dir >= -1; dir -= 2) {
+        max_angle = -2000;
+        visible = 1;
+        open_delta_d = 0;
+        for (int k = j + dir; k >= inicio && k < final; k += dir) {
+            delta_d = (double)(dir > 0 ? k - j : j - k);
+            cAngle = (row[k] - h) / delta_d;
+            above = cAngle > max_angle;
+            if (above && !visible) open_delta_d = delta_d;
+            if (!above && visible) cv += delta_d * delta_d - open_delta_d * open_delta_d;
+            visible = above;
+            if (cAngle > max_angle) max_angle = cAngle;
+        }
+    }
+    return cv;
+}
+
+int orc_execute_sdem_f64(int dimx, int dimy, const float *input, double *out, int nthreads, int s0, int s1,
+                         double povh)
+{
+    size_t N = (size_t)dimx * dimy;
+    float *in1 = (float *)malloc(N * sizeof(float));
+    float *in2 = (float *)malloc(N * sizeof(float));
+    float *in3 = (float *)malloc(N * sizeof(float));
+    if (!in1 || !in2 || !in3) return -1;
+    orc_prepare(input, dimx, dimy, in1, in2, in3);
+    const float *in4[4] = {input, in1, in2, in3};
+    memset(out, 0, N * sizeof(double));
+    if (nthreads <= 0) nthreads = orc_default_threads();
+    int dim_l = dimx > dimy ? dimx : dimy;
+    int err = 0;
+#pragma omp parallel num_threads(nthreads)
+    {
+        int *target = (int *)calloc((size_t)dim_l, sizeof(int));
+        float *weight = (float *)calloc((size_t)dim_l, sizeof(float));
+        double *wd = (double *)calloc((size_t)dim_l, sizeof(double));
+        uint16_t *first = (uint16_t *)calloc(2 * (size_t)dim_l + 1, sizeof(uint16_t));
+        uint16_t *last = (uint16_t *)calloc(2 * (size_t)dim_l + 1, sizeof(uint16_t));
+        double *S = (double *)malloc((2 * N + dim_l) * sizeof(double));
+        double *SO = (double *)malloc((2 * N + dim_l) * sizeof(double));
+        double *acc[4] = {NULL, NULL, NULL, NULL};
+        if (!target || !weight || !wd || !first || !last || !S || !SO) {
+#pragma omp atomic write
+            err = -1;
+        }
+#pragma omp barrier
+#pragma omp for schedule(dynamic) nowait
+        for (int a = s0; a < s1; a++) {
+            if (err) continue;
+            orc_params p;
+            orc_compute_params(dimx, dimy, a, &p, target, weight);
+            orc_skew_shape(&p, target, first, last);
+            int di = p.dim_i, H = p.skewHeight;
+            for (int j = 0; j < di; j++) wd[j] = p.skewness * j - target[j];
+            memset(S, 0, (size_t)H * di * sizeof(double));
+            memset(SO, 0, (size_t)H * di * sizeof(double));
+            const float *src = in4[p.sectorType];
+            for (int i = 0; i < p.dim_o; i++)
+                for (int j = 0; j < di; j++) {
+                    double v = src[(size_t)di * i + j];
+                    S[(size_t)(i + target[j]) * di + j] += (1.0 - wd[j]) * v;
+                    S[(size_t)(i + target[j] + 1) * di + j] += wd[j] * v;
+                }
+            double r = p.scale * p.scale;
+            for (int i = 0; i < H; i++)
+                for (int j = first[i]; j < last[i]; j++)
+                    SO[(size_t)di * i + j] = observer_f64(S + (size_t)di * i, first[i], last[i], j, povh) * r;
+            int t = p.sectorType;
+            if (!acc[t]) acc[t] = (double *)calloc(N, sizeof(double));
+            if (!acc[t]) {
+#pragma omp atomic write
+                err = -1;
+                continue;
+            }
+            for (int i = 0; i < p.dim_o; i++)
+                for (int j = 0; j < di; j++)
+                    acc[t][(size_t)i * di + j] += (1.0 - wd[j]) * SO[(size_t)(i + target[j]) * di + j] +
+                                                  wd[j] * SO[(size_t)(i + target[j] + 1) * di + j];
+        }
+#pragma omp critical
+        {
+            if (!err)
+                for (int i = 0; i < dimy; i++) {
+                    int ci = dimy - 1 - i;
+                    for (int j = 0; j < dimx; j++) {
+                        int cj = dimx - 1 - j;
+                        double *o = out + (size_t)i * dimx + j;
+                        if (acc[0]) *o += acc[0][(size_t)i * dimx + j];
+                        if (acc[1]) *o += acc[1][(size_t)cj * dimy + ci];
+                        if (acc[2]) *o += acc[2][(size_t)j * dimy + ci];
+                        if (acc[3]) *o += acc[3][(size_t)i * dimx + cj];
+                    }
+                }
+        }
+        for (int t = 0; t < 4; t++) free(acc[t]);
+        free(target); free(weight); free(wd); free(first); free(last); free(S); free(SO);
+    }
     free(in1); free(in2); free(in3);
     return err;
 }

@@ -64,11 +64,17 @@ void orc_op_radon_local(const orc_params *p, const uint16_t *first, const uint16
  * receives the squared magnitudes the peak rule scans. */
 double orc_point_radon(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
                        int i, int j, double *spec_out);
+/* The same cell, returning the four 64-float vectors the reference records
+ * for a sample point (kernelRADN.h:164-170): windowed sums, axis, filtered
+ * spectrum, real part of its transform. */
+double orc_point_radon_samples(const orc_params *p, const uint16_t *first, const uint16_t *last, const float *S,
+                               int i, int j, float *samples_out);
 double orc_blur_parameter2(const double *v, int n, int *imax);
 float orc_line_cepstrum(int j, const float *row, int first, int last);
 /* The same, also returning the intermediates: the 32 log powers ln|X_k|^2
- * and the 16 samples -min(0, Re C_k) the peak rule scans (either may be NULL). */
-float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *samples_out);
+ * and the 16 cepstrum values Re C_k whose -min(0, .) the peak rule scans
+ * (either pointer may be NULL). */
+float orc_line_cepstrum_ex(int j, const float *row, int first, int last, double *logps_out, double *ceps_out);
 
 /* One observer's two line-of-sight scans on a staged row; writes the
  * visibility decision of every step (1 = above the horizon) and returns cv. */
@@ -82,6 +88,11 @@ int orc_default_threads(void);
  * main.cu:187-192.  Returns 0 on success. */
 int orc_execute(int alg, int dimx, int dimy, const float *input, float *out,
                 int nthreads, int s0, int s1, float povh, double *seconds);
+
+/* FP64 variant of the total-viewshed farm over sectors [s0, s1): same maps
+ * and algorithm, every value in double (out: dimx*dimy doubles). */
+int orc_execute_sdem_f64(int dimx, int dimy, const float *input, double *out, int nthreads, int s0, int s1,
+                         double povh);
 
 /* Run one sector through skew -> operator on a fresh engine and return its
  * tables and skewed buffers (sized 2*N + dim_l floats each).  For tests. */

@@ -6,6 +6,6 @@ the synthetic-input generators.
 """
 from .engine import (  # noqa: F401
     KERNEL_CEPSTRUM, KERNEL_IDENTITYDEM, KERNEL_IDENTITYIMG, KERNEL_RADON, KERNEL_SDEM, POVH_DEFAULT,
-    SkewBatch, SkewEngine, device_count, execute, plan_sector, run_cli, work,
+    SkewBatch, SkewEngine, device_count, execute, plan_sector, run_cli, work, write_sample_files,
 )
 from ._lib import SkeError, lib  # noqa: F401

@@ -16,8 +16,9 @@ OPT_SDEM_OCCUPANCY = 7
 OPT_RADON_LOCAL = 8
 OPT_COUNT_STEPS = 9
 OPT_SDEM_GROUP = 10
-LEARNED_TAPS = 4945
-LEARNED_CHANNELS = 16
+LEARNED_TAPS = 4945        # offset of the filter taps, 16 hidden channels
+LEARNED_TAPS_C32 = 19105   # the same, 32 hidden channels
+LEARNED_CHANNELS = 16      # default hidden channels
 LEARNED_R0, LEARNED_A1, LEARNED_A2, LEARNED_A3, LEARNED_DR0, LEARNED_FILTERED_GRAD = 0, 1, 2, 3, 4, 5
 MAX_SECTOR = 180
 
@@ -25,7 +26,8 @@ MAX_SECTOR = 180
 EXPORTS = [
     "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_create",
     "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
-    "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_plan_sector",
+    "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_plan_sector", "ske_set_sample_points",
+    "ske_get_sample_data", "ske_write_sample_files",
     "ske_reset_skew", "ske_get_skew", "ske_set_skew_output",
     "ske_device_pointers", "ske_run", "ske_output_device", "ske_synchronize", "ske_set_option", "ske_get_option",
     "ske_profile_get", "ske_counter_get", "ske_profile_reset", "ske_work",
@@ -36,7 +38,8 @@ EXPORTS = [
     "ske_fbp_create", "ske_fbp_destroy", "ske_fbp_upload", "ske_fbp_download", "ske_fbp_buffer", "ske_fbp_project",
     "ske_fbp_filter", "ske_fbp_backproject", "ske_fbp_reconstruct", "ske_fbp_set_taps", "ske_fbp_get_taps",
     "ske_fbp_synchronize", "ske_fbp_set_option", "ske_fbp_profile_get", "ske_fbp_profile_reset",
-    "ske_learned_create", "ske_learned_destroy", "ske_learned_param_count", "ske_learned_init",
+    "ske_learned_create", "ske_learned_create_channels", "ske_learned_destroy", "ske_learned_param_count",
+    "ske_learned_layout", "ske_learned_init",
     "ske_learned_set_params", "ske_learned_get_params", "ske_learned_reconstruct", "ske_learned_gradient",
     "ske_learned_train_step", "ske_learned_download", "ske_random_ellipses", "ske_learned_render",
     "ske_learned_make_batch", "ske_png_read", "ske_png_write", "ske_result_pngs",
@@ -94,6 +97,9 @@ def lib():
     L.ske_get_tables.argtypes = [vp, vp, vp, vp, vp]
     L.ske_plan_sector.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(SectorParams), vp, vp, vp, vp]
     L.ske_reset_skew.argtypes = [vp]
+    L.ske_set_sample_points.argtypes = [vp, ctypes.c_int, vp]
+    L.ske_get_sample_data.argtypes = [vp, vp]
+    L.ske_write_sample_files.argtypes = [ctypes.c_char_p, ctypes.c_int, vp]
     L.ske_get_skew.argtypes = [vp, vp, vp]
     L.ske_set_skew_output.argtypes = [vp, vp]
     L.ske_device_pointers.argtypes = [vp, P(vp), P(vp), P(vp), P(vp)]
@@ -136,6 +142,8 @@ def lib():
     L.ske_png_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     L.ske_result_pngs.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp]
     L.ske_learned_create.argtypes = [vp, P(vp)]
+    L.ske_learned_create_channels.argtypes = [vp, ctypes.c_int, P(vp)]
+    L.ske_learned_layout.argtypes = [vp, i32p, i32p]
     L.ske_learned_destroy.argtypes = [vp]
     L.ske_learned_param_count.argtypes = [vp, i32p]
     L.ske_learned_init.argtypes = [vp, ctypes.c_ulonglong]

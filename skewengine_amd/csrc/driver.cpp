@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "host_util.h"
 #include "skewengine.h"
 
 namespace {
@@ -106,6 +107,26 @@ int kernel_from_name(const char* s) {
     return -1;
 }
 
+// One sample file: the reference writes, per sector, values [32 + j] of the
+// vector (j = 0..31), then per sector values [32 - j]; for the spectrum file
+// ('l') both blocks use [32 - j] (src/kernelRADN.h:280-360).  Each value is
+// followed by a space (std::ostream default float format), each line by \n.
+bool write_sample_file(const std::string& path, int count, int p, int kind, const float* data) {
+    FILE* f = fopen(path.c_str(), "w");
+    if (!f) return false;
+    for (int block = 0; block < 2; block++)
+        for (int a = 0; a < SKE_MAX_SECTOR; a++) {
+            const float* v = data + ((size_t)a * count + p) * 256 + (size_t)kind * 64;
+            for (int j = 0; j < 32; j++) {
+                const int idx = (block == 0 && kind != 2) ? 32 + j : 32 - j;
+                fprintf(f, "%g ", (double)v[idx]);
+            }
+            fputc('\n', f);
+        }
+    const bool ok = ferror(f) == 0;
+    return fclose(f) == 0 && ok;
+}
+
 // Largest grid side the engine accepts (first/last are 16-bit, one row plus
 // guard bands must fit the LDS of a workgroup).
 constexpr int kMaxSide = 16000;
@@ -114,6 +135,8 @@ int run_impl(int argc, char* argv[]) {
     int kernel = SKE_KERNEL_SDEM;
     int local = 0;
     std::string input, output, png_dir;
+    std::string sample_dir;
+    std::vector<int> samples = {1082, 562, 348, 266, 600, 536};  // punto1..3 (src/global.cpp:54-56)
     int dimx = 0, dimy = 0, s0 = 0, s1 = SKE_MAX_SECTOR, device = 0;
     float povh = (float)(1.5 / 10.0);  // POVh = obsheight/step (src/kernelSDEM.h:159)
     for (int i = 1; i < argc; i++) {
@@ -145,6 +168,24 @@ int run_impl(int argc, char* argv[]) {
             i++;
         } else if (!strcmp(a, "--local")) {  // the reference's LRT flag (src/kernelRADN.h:212)
             local = 1;
+        } else if (!strcmp(a, "--sample-dir") && v) {  // saveSampleData = true, dumps into this directory
+            sample_dir = v;
+            i++;
+        } else if (!strcmp(a, "--samples") && v) {  // "x,y;x,y;..." replaces the three default points
+            samples.clear();
+            const char* q = v;
+            int x = 0, y = 0, used = 0;
+            while (*q && sscanf(q, "%d,%d%n", &x, &y, &used) == 2) {
+                samples.push_back(x);
+                samples.push_back(y);
+                q += used;
+                if (*q == ';') q++;
+            }
+            if (*q || samples.empty() || samples.size() / 2 > 16) {
+                fprintf(stderr, "run: --samples wants 1 to 16 points as x,y;x,y;...\n");
+                return SKE_EINVAL;
+            }
+            i++;
         } else {
             fprintf(stderr, "run: unknown or incomplete argument '%s'\n", a);
             return SKE_EINVAL;
@@ -213,9 +254,19 @@ int run_impl(int argc, char* argv[]) {
     auto t1 = std::chrono::steady_clock::now();
     ske_engine* e = nullptr;
     int rc = ske_create(dimx, dimy, grid.data(), device, &e);
+    // the reference writes sample files from showResultsRADN, i.e. for the
+    // Radon and identity-image operators (src/main.h:137-138)
+    const bool dump = !sample_dir.empty() && (kernel == SKE_KERNEL_RADON || kernel == SKE_KERNEL_IDENTITYIMG);
+    const int npts = (int)samples.size() / 2;
+    std::vector<float> record;
     if (rc == SKE_OK) {
         if (local) rc = ske_set_option(e, SKE_OPT_RADON_LOCAL, 1);
+        if (rc == SKE_OK && dump) rc = ske_set_sample_points(e, npts, samples.data());
         if (rc == SKE_OK) rc = ske_run(e, kernel, s0, s1, povh, out.data());
+        if (rc == SKE_OK && dump) {
+            record.assign((size_t)SKE_MAX_SECTOR * npts * 256, 0.0f);
+            rc = ske_get_sample_data(e, record.data());
+        }
         ske_destroy(e);
     }
     auto t2 = std::chrono::steady_clock::now();
@@ -239,6 +290,13 @@ int run_impl(int argc, char* argv[]) {
         }
         fclose(f);
     }
+    if (dump) {
+        rc = ske_write_sample_files(sample_dir.c_str(), npts, record.data());
+        if (rc != SKE_OK) {
+            fprintf(stderr, "run: %s\n", ske_last_error());
+            return rc;
+        }
+    }
     if (!png_dir.empty()) {
         rc = ske_result_pngs(png_dir.c_str(), kernel, dimx, dimy, grid.data(), out.data());
         if (rc != SKE_OK) {
@@ -258,4 +316,21 @@ extern "C" int run(int argc, char* argv[]) {
         fprintf(stderr, "run: out of host memory\n");
         return SKE_ENOMEM;
     }
+}
+
+extern "C" int ske_write_sample_files(const char* dir, int count, const float* data) {
+    if (!dir || !data || count < 1 || count > 16)
+        return ske::fail(SKE_EINVAL, "ske_write_sample_files: dir/data NULL or count outside [1,16]");
+    static const char kinds[4] = {'o', 'i', 'l', 'c'};
+    try {
+        for (int p = 0; p < count; p++)
+            for (int k = 0; k < 4; k++) {
+                std::string path = std::string(dir) + "/punto" + std::to_string(p + 1) + kinds[k] + ".txt";
+                if (!write_sample_file(path, count, p, k, data))
+                    return ske::fail(SKE_EINVAL, "ske_write_sample_files: cannot write " + path);
+            }
+    } catch (...) {
+        return ske::fail(SKE_ENOMEM, "ske_write_sample_files: out of host memory");
+    }
+    return SKE_OK;
 }

@@ -84,7 +84,37 @@ struct Chunk {
         int range[8];
     };
     std::vector<Part> parts;
+    DevBuf<SampleReq> sreq;  // sample points of the local Radon transform in this chunk
+    int nreq = 0;
+    int sample_version = -1;  // Core::sample_version the requests were built for
 };
+
+// Sample-point requests of one planned chunk: for every sector of grid
+// `img_off` (four-copy offset) and every point (x, y), the skewed cell whose
+// source is (x, y) -- getXY (src/kernelRADN.h:28-41) inverted -- when it lies
+// inside its row's [first, last), the cells pointRadon visits.
+void sample_requests(const ImagePlan& ip, long long img_off, int n, const std::vector<int>& xy,
+                     std::vector<SampleReq>& out) {
+    out.clear();
+    const int npts = (int)xy.size() / 2;
+    for (int s = 0; s < (int)ip.secs.size(); s++) {
+        const SecDesc& sd = ip.secs[s];
+        if (sd.img != img_off) continue;
+        const bool tr = sd.type == 1 || sd.type == 2, rx = sd.type == 1 || sd.type == 3,
+                   ry = sd.type == 1 || sd.type == 2;
+        for (int p = 0; p < npts; p++) {
+            const int x = xy[2 * p], y = xy[2 * p + 1];
+            if (x < 0 || x >= n || y < 0 || y >= n) continue;
+            const int xx = rx ? n - 1 - x : x, yy = ry ? n - 1 - y : y;
+            const int j = tr ? yy : xx;
+            const int i = (tr ? xx : yy) + ip.target[sd.tab + j];
+            if (i < 0 || i >= sd.H) continue;
+            const uint32_t v = ip.fl[sd.rows + i];
+            if (j < (int)(v & 0xffffu) || j >= (int)(v >> 16)) continue;
+            out.push_back({s, i, j, sd.angle * npts + p});
+        }
+    }
+}
 
 struct RunPlan {
     int s0 = 0, s1 = 0, order = 0;
@@ -145,6 +175,12 @@ struct Core {
     bool use_graph = false;
     bool waves_set = false;  // waves per workgroup chosen by the caller or the autotuner
     bool count_steps = false;  // viewshed: count executed lane-steps (counter[2..3] as a u64)
+    // sample points of the local Radon transform (saveSampleData,
+    // src/global.cpp:47-56): image cells (x, y) of grid 0, and the recorded
+    // vectors, 180 sectors x points x 4 x 64 floats
+    std::vector<int> sample_xy;
+    int sample_version = 0;
+    DevBuf<float> sout;
     std::map<std::string, double> counters;
     // profiling
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
@@ -373,6 +409,20 @@ struct Core {
             if (arena.n < (size_t)plan->max_so || W.n < (size_t)plan->max_so) drop_graph();
             SKE_TRY(arena.ensure((size_t)plan->max_so));
             SKE_TRY(W.ensure((size_t)plan->max_so));
+            if (!sample_xy.empty()) {
+                const size_t nout = (size_t)SKE_MAX_SECTOR * (sample_xy.size() / 2) * 256;
+                if (sout.n < nout) drop_graph();
+                SKE_TRY(sout.ensure(nout));
+                for (auto& cp : plan->chunks) {
+                    if (cp->sample_version == sample_version) continue;
+                    std::vector<SampleReq> req;
+                    sample_requests(cp->plan.ip, imgs[0].in_off, imgs[0].n, sample_xy, req);
+                    if (!req.empty()) SKE_TRY(cp->sreq.upload(req.data(), req.size(), stream));
+                    cp->nreq = (int)req.size();
+                    cp->sample_version = sample_version;
+                }
+                SKE_HIP(hipStreamSynchronize(stream));  // the host request vectors are gone
+            }
         } else if (kernel_id == SKE_KERNEL_RADON) {
             if (next_pow2_log(plan->maxH) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
             if (P.n < (size_t)plan->max_q || q.n < (size_t)plan->max_q) drop_graph();
@@ -407,6 +457,11 @@ struct Core {
         const int is_max = kernel_id == SKE_KERNEL_CEPSTRUM ? 1 : 0;
         const size_t acc_n = acc.n;
         SKE_TRY(launch("fill", [&] { return launch_fill_f32(acc.p, acc_n, 0.0f, stream); }));
+        const bool samples = local && !sample_xy.empty();
+        if (samples) {
+            const size_t nout = (size_t)SKE_MAX_SECTOR * (sample_xy.size() / 2) * 256;
+            SKE_TRY(launch("fill", [&] { return launch_fill_f32(sout.p, nout, 0.0f, stream); }));
+        }
         for (auto& cp : plan->chunks) {
             const DevPlan v = cp->plan.view();
             const ImagePlan& ip = cp->plan.ip;
@@ -433,6 +488,12 @@ struct Core {
                     SKE_TRY(launch("op_lrt", [&] {
                         return launch_lrt(v, in4.p, nullptr, W.p, arena.p, ctab.p, gtab.p, stream);
                     }));
+                    if (samples && cp->nreq) {
+                        SKE_TRY(launch("lrt_samples", [&] {
+                            return launch_lrt_samples(v, cp->sreq.p, cp->nreq, in4.p, W.p, ctab.p, gtab.p, sout.p,
+                                                      stream);
+                        }));
+                    }
                     break;
                 }
                 SKE_TRY(launch("fill", [&] { return launch_fill_f64(P.p, (size_t)ip.q_total, stream); }));
@@ -472,9 +533,9 @@ struct Core {
         memcpy(&pbits, &povh, sizeof pbits);
         // every setting that changes what enqueue_run records is part of the key
         // (waves_set: unset wave counts are chosen per chunk by effective_tuning)
-        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%d:%d:%lld:%d:%d", kernel_id, s0, s1, pbits,
+        snprintf(key, sizeof key, "%d:%d:%d:%08x:%d:%d:%d:%d:%d:%d:%lld:%d:%d:%d", kernel_id, s0, s1, pbits,
                  sdem_variant, tune.waves, waves_set ? 1 : 0, tune.persistent_wgs_per_cu, tune.occupancy, tune.group,
-                 arena_bytes, radon_local ? 1 : 0, count_steps ? 1 : 0);
+                 arena_bytes, radon_local ? 1 : 0, count_steps ? 1 : 0, sample_version);
         if (!gexec || gkey != key) {
             drop_graph();
             SKE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
@@ -654,6 +715,7 @@ struct ske_engine {
     DevBuf<double> P1;
     DevBuf<float> q1;
     DevBuf<double> W1;  // local Radon window sums of the current sector
+    DevBuf<SampleReq> sreq1;  // its sample-point requests
 };
 
 namespace {
@@ -741,6 +803,23 @@ int sector_kernel(ske_engine* e) {
             SKE_TRY(c.launch("op_lrt", [&] {
                 return launch_lrt(v, c.in4.p, e->S.p, e->W1.p, e->SO.p, c.ctab.p, c.gtab.p, c.stream);
             }));
+            if (!c.sample_xy.empty()) {
+                const size_t nout = (size_t)SKE_MAX_SECTOR * (c.sample_xy.size() / 2) * 256;
+                if (c.sout.n < nout) {
+                    SKE_TRY(c.sout.ensure(nout));
+                    SKE_TRY(c.launch("fill", [&] { return launch_fill_f32(c.sout.p, nout, 0.0f, c.stream); }));
+                }
+                std::vector<SampleReq> req;
+                sample_requests(e->one.ip, 0, e->dimx, c.sample_xy, req);
+                if (!req.empty()) {
+                    SKE_TRY(e->sreq1.upload(req.data(), req.size(), c.stream));
+                    SKE_TRY(c.launch("lrt_samples", [&] {
+                        return launch_lrt_samples(v, e->sreq1.p, (int)req.size(), c.in4.p, e->W1.p, c.ctab.p,
+                                                  c.gtab.p, c.sout.p, c.stream);
+                    }));
+                    SKE_HIP(hipStreamSynchronize(c.stream));  // req goes out of scope
+                }
+            }
             break;
         }
         if (next_pow2_log(sp.H) > kMaxFftLog) return fail(SKE_EUNSUPPORTED, "Radon projection longer than 4096 bins");
@@ -971,6 +1050,37 @@ int ske_work(int dimx, int dimy, int s0, int s1, double* cells, double* steps) {
     plan_image(dimx, dimy, s0, s1, 0, 0, 0, kOrderNatural, ip);
     if (cells) *cells = ip.cells;
     if (steps) *steps = ip.steps;
+    return SKE_OK;
+}
+
+int ske_set_sample_points(ske_engine* e, int count, const int* xy) {
+    if (!e) return fail(SKE_EINVAL, "engine is NULL");
+    if (count < 0 || count > kMaxSamplePoints)
+        return fail(SKE_EINVAL, "sample point count must be in [0, " + std::to_string(kMaxSamplePoints) + "]");
+    if (count > 0 && !xy) return fail(SKE_EINVAL, "xy is NULL");
+    Core& c = e->core;
+    SKE_TRY(c.sync());
+    c.sample_xy.assign(xy, xy + 2 * count);
+    c.sample_version++;
+    c.drop_graph();
+    if (count) {  // a fresh, zeroed record
+        const size_t nout = (size_t)SKE_MAX_SECTOR * count * 256;
+        SKE_HIP(hipSetDevice(c.device));
+        c.sout.release();
+        SKE_TRY(c.sout.ensure(nout));
+        SKE_TRY(c.launch("fill", [&] { return launch_fill_f32(c.sout.p, nout, 0.0f, c.stream); }));
+        SKE_TRY(c.sync());
+    }
+    return SKE_OK;
+}
+
+int ske_get_sample_data(ske_engine* e, float* out) {
+    if (!e || !out) return fail(SKE_EINVAL, "engine or out is NULL");
+    Core& c = e->core;
+    if (c.sample_xy.empty()) return fail(SKE_ESTATE, "no sample points set");
+    SKE_TRY(c.sync());
+    const size_t nout = (size_t)SKE_MAX_SECTOR * (c.sample_xy.size() / 2) * 256;
+    SKE_HIP(hipMemcpy(out, c.sout.p, nout * sizeof(float), hipMemcpyDeviceToHost));
     return SKE_OK;
 }
 

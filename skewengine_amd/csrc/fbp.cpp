@@ -47,7 +47,6 @@ extern "C" {
 int ske_fbp_create(int n, int n_angles, int n_det, const float* angles, int max_batch, int device_id, ske_fbp** out) {
     if (!out) return fail(SKE_EINVAL, "out is NULL");
     *out = nullptr;
-    // the filter keeps a row and its 2*n_det-1 taps in LDS (<= 64 KB)
     if (n < 2 || n > 8192 || n_angles < 1 || n_angles > 4096 || n_det < 2 || n_det > 5000 || max_batch < 1)
         return fail(SKE_EINVAL, "FBP geometry out of range (n <= 8192, n_angles <= 4096, n_det <= 5000)");
     int ndev = 0;

@@ -18,8 +18,8 @@ struct FbpGeom {
 
 // Zero border of the projector's image copy (padded: batch x (n + 2 kProjPad)^2).
 constexpr int kProjPad = 4;
-// ang_bins: angles with |cos| >= |sin| (lanes over bins); ang_rays: the
-// others (lanes over the samples of one ray).
+// ang_bins: angles with |cos| >= |sin|; ang_rays: the others (the two lists
+// together cover every angle once).
 hipError_t launch_fbp_project(const float* img, float* padded, float* sino, const float2* cs, const int* ang_bins,
                               int n_bins, const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s);
 hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, const FbpGeom& g, int batch,

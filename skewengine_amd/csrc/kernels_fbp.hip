@@ -8,14 +8,15 @@
 //
 //   project     p(a,k) = delta * sum_t f(s_k cos - t sin, s_k sin + t cos),
 //               t on a delta grid over the chord of the ray through the
-//               image square, f bilinear (zero outside); lanes run along
-//               whichever of bins or samples keeps the gathers on one image
-//               row;
+//               image square, f bilinear (zero outside); a wave's lanes
+//               cover 8 bins x 8 samples, a compact block of the image;
 //   filter      q(a,k) = sum_k' p(a,k') h(k-k'), h = delta * Ram-Lak
 //               (h0 = 1/(4 delta^2), odd m: -1/(m^2 pi^2 delta^2), even: 0) or
-//               learned taps;
+//               learned taps, as a GEMM with a Toeplitz matrix on the FP32
+//               matrix cores;
 //   backproject f(x,y) = (pi/nang) sum_a q(a, s(x,y)), linear in s.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 
@@ -92,112 +93,123 @@ __device__ __forceinline__ float ray_sample(const float* __restrict__ pim, int n
     return bilinear_padded(pim, np, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
 }
 
-// Angles with |cos| >= |sin|: one thread per (bin, angle, slice) summing its
-// ray's samples in order.  Neighbouring lanes (bins) sample points
-// delta (cos, sin) apart: nearly the same image row, so the gathers
-// coalesce.  `ang` lists the angle indices this launch covers.
-__global__ __launch_bounds__(256) void k_fbp_project_bins(const float* __restrict__ pimg, float* __restrict__ sino,
-                                                          const float2* __restrict__ cs, const int* __restrict__ ang,
-                                                          FbpGeom g) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = ang[blockIdx.y];
-    const int b = blockIdx.z;
-    if (k >= g.ndet) return;
-    const float2 t2 = cs[a];
-    const float s = ((float)k - g.det_c) * g.delta;
-    int i0, i1;
-    ray_range(g, s, t2.x, t2.y, i0, i1);
-    const int np = g.n + 2 * kProjPad;
-    const float* im = pimg + (size_t)b * np * np;
-    // four samples' gathers in flight per pass, summed in sample order
-    float acc = 0.0f;
-    int i = i0;
-    for (; i + 3 <= i1; i += 4) {
-        float v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = ray_sample(im, np, g, s, t2.x, t2.y, i + u);
-        acc += v[0];
-        acc += v[1];
-        acc += v[2];
-        acc += v[3];
-    }
-    for (; i <= i1; ++i) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
-    sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
-}
-
-// Angles with |sin| > |cos|: one wave per ray, lanes over its samples
-// (neighbouring samples are delta (-sin, cos) apart: again nearly one image
-// row), then a wave reduction.
-__global__ __launch_bounds__(256) void k_fbp_project_rays(const float* __restrict__ pimg, float* __restrict__ sino,
-                                                          const float2* __restrict__ cs, const int* __restrict__ ang,
-                                                          FbpGeom g) {
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+// One wave per 8 bins of one angle and slice: lane (kb, ib) = (lane % 8,
+// lane / 8) takes the samples i = i_lo + ib, i_lo + ib + 8, ... of bin
+// k0 + kb, where [i_lo, i_hi] covers the 8 bins' sample ranges; samples
+// outside the lane's own range [i0, i1] are skipped (they would add exact
+// zeros, and may lie beyond the zero border).  At every pass the 64 lanes'
+// samples form an 8 x 8 block of (bin, sample) points, a rotated square of
+// side 8 delta: their bilinear taps fall on a handful of image rows and
+// cache lines at every angle.  Four passes' gathers are in flight together;
+// each lane sums its samples in order, then the 8 lanes of a bin are added
+// in a fixed butterfly order.
+__global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ pimg, float* __restrict__ sino,
+                                                     const float2* __restrict__ cs, const int* __restrict__ ang,
+                                                     FbpGeom g) {
     const int lane = threadIdx.x & 63;
+    const int kb = lane & 7, ib = lane >> 3;
+    const int kbase = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 8;
     const int a = ang[blockIdx.y];
     const int b = blockIdx.z;
-    if (k >= g.ndet) return;
+    if (kbase >= g.ndet) return;  // uniform per wave
+    const int k = min(kbase + kb, g.ndet - 1);
     const float2 t2 = cs[a];
     const float s = ((float)k - g.det_c) * g.delta;
     int i0, i1;
     ray_range(g, s, t2.x, t2.y, i0, i1);
+    if (kbase + kb >= g.ndet) i1 = i0 - 1;  // padding lanes: no samples
+    // common range of the wave's 8 bins (every lane holds its own bin's range)
+    int lo = i1 >= i0 ? i0 : INT_MAX, hi = i1 >= i0 ? i1 : INT_MIN;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
     const int np = g.n + 2 * kProjPad;
     const float* im = pimg + (size_t)b * np * np;
     float acc = 0.0f;
-    int i = i0 + lane;
-    for (; i + 64 <= i1; i += 128) {
-        const float v0 = ray_sample(im, np, g, s, t2.x, t2.y, i);
-        const float v1 = ray_sample(im, np, g, s, t2.x, t2.y, i + 64);
-        acc += v0;
-        acc += v1;
-    }
-    if (i <= i1) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
+    if (lo <= hi) {
+        int i = lo + ib;
+        for (; i + 24 <= hi; i += 32) {
+            float v[4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
+            for (int u = 0; u < 4; ++u) {
+                const int iu = i + 8 * u;
+                v[u] = 0.0f;
+                if (iu >= i0 && iu <= i1) v[u] = ray_sample(im, np, g, s, t2.x, t2.y, iu);  // lanes masked: no load
+            }
+            acc += v[0];
+            acc += v[1];
+            acc += v[2];
+            acc += v[3];
+        }
+        for (; i <= hi; i += 8)
+            if (i >= i0 && i <= i1) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
+    }
+    acc += __shfl_xor(acc, 8, 64);
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (ib == 0 && kbase + kb < g.ndet) sino[((size_t)b * g.nang + a) * g.ndet + kbase + kb] = acc * g.delta;
 }
 
-// One workgroup per sinogram row: the row and the 2*ndet-1 filter taps in
-// LDS.  A thread owns 4 consecutive outputs and slides a 4-tap register
-// window along h, so each step costs one broadcast read of p, one read of h
-// and 4 FMAs; every output still sums k' = 0, 1, ... in order.
-__global__ __launch_bounds__(256) void k_fbp_filter(const float* __restrict__ sino, float* __restrict__ filt,
-                                                    const float* __restrict__ taps, int ndet) {
-    extern __shared__ float sm[];
-    float* p = sm;
-    float* h = sm + ndet;
-    const size_t row = blockIdx.x;
-    const float* src = sino + row * ndet;
-    for (int i = threadIdx.x; i < ndet; i += blockDim.x) p[i] = src[i];
-    for (int i = threadIdx.x; i < 2 * ndet - 1; i += blockDim.x) h[i] = taps[i];
-    __syncthreads();
-    for (int k = 4 * (int)threadIdx.x; k < ndet; k += 4 * (int)blockDim.x) {
-        // outputs k..k+3 (those >= ndet are computed and dropped); h index
-        // of output k+m at k' is k + m - k' + ndet - 1, in [0, 2 ndet - 1)
-        // for k + m < ndet; taps past the end read as 0
-        const int top = 2 * ndet - 1;
-        float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
-        int hb = k + ndet - 1;  // h index of output k at k' = 0
-        float w0 = h[hb];
-        float w1 = hb + 1 < top ? h[hb + 1] : 0.0f;
-        float w2 = hb + 2 < top ? h[hb + 2] : 0.0f;
-        float w3 = hb + 3 < top ? h[hb + 3] : 0.0f;
-        for (int kp = 0; kp < ndet; ++kp) {
-            const float x = p[kp];
-            acc0 = __builtin_fmaf(x, w0, acc0);
-            acc1 = __builtin_fmaf(x, w1, acc1);
-            acc2 = __builtin_fmaf(x, w2, acc2);
-            acc3 = __builtin_fmaf(x, w3, acc3);
-            w3 = w2;
-            w2 = w1;
-            w1 = w0;
-            --hb;
-            w0 = hb >= 0 ? h[hb] : 0.0f;
+// The filter as a GEMM on the matrix cores: Q = P T over the sinogram rows
+// P (rows x ndet), with the Toeplitz T[k'][k] = h[k - k' + ndet - 1], in
+// FP32 (v_mfma_f32_16x16x4_f32: fp32 products, fp32 accumulation).  A
+// workgroup owns 16 rows and kFiltTiles 16-column output tiles; each of its
+// waves keeps kFiltNB tiles in flight, so one A fragment feeds kFiltNB
+// MFMAs.  K runs in chunks of up to kFiltKC: the chunk's 16 rows (zero past
+// ndet and past the last row) and the window of taps its B fragments read
+// (zero outside [0, 2 ndet - 1)) are staged in LDS.  Lane layout: A[i][k]
+// at lane i + 16k, B[k][j] at lane j + 16k, D[i][j] at lane j + 16(i/4),
+// register i%4.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kFiltNB = 4;                    // output tiles per wave pass
+constexpr int kFiltTiles = 4 * kFiltNB;       // output tiles per workgroup (4 waves)
+constexpr int kFiltKC = 768;                  // K chunk (a multiple of 4)
+__global__ __launch_bounds__(256) void k_fbp_filter_mfma(const float* __restrict__ sino, float* __restrict__ filt,
+                                                         const float* __restrict__ taps, int ndet, int rows,
+                                                         int kpad) {
+    __shared__ float P[16 * kFiltKC];                 // rows r0.., columns kc..kc+kl
+    __shared__ float hw[kFiltTiles * 16 + kFiltKC];   // taps window of the chunk
+    const int r0 = blockIdx.x * 16;
+    const int c0 = blockIdx.y * kFiltTiles * 16;  // first output column of this workgroup
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kk = lane >> 4;
+    f32x4 acc[kFiltNB];
+#pragma unroll
+    for (int u = 0; u < kFiltNB; u++) acc[u] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kc = 0; kc < kpad; kc += kFiltKC) {
+        const int kl = min(kFiltKC, kpad - kc);
+        __syncthreads();  // the previous chunk's reads are done
+        for (int e = threadIdx.x; e < 16 * kl; e += 256) {
+            const int r = e / kl, k = e - r * kl;
+            P[r * kFiltKC + k] = (r0 + r < rows && kc + k < ndet) ? sino[(size_t)(r0 + r) * ndet + kc + k] : 0.0f;
         }
-        float* o = filt + row * ndet;
-        o[k] = acc0;
-        if (k + 1 < ndet) o[k + 1] = acc1;
-        if (k + 2 < ndet) o[k + 2] = acc2;
-        if (k + 3 < ndet) o[k + 3] = acc3;
+        // column n = c0 + m (m < 16 kFiltTiles) at k' = kc + q (q < kl) needs
+        // h[n - k' + ndet - 1], stored at hw[m - q + kl - 1]
+        const int hbase = c0 + ndet - kc - kl;
+        for (int y = threadIdx.x; y < kFiltTiles * 16 + kl; y += 256) {
+            const int idx = hbase + y;
+            hw[y] = (idx >= 0 && idx < 2 * ndet - 1) ? taps[idx] : 0.0f;
+        }
+        __syncthreads();
+        const float* pa = P + i * kFiltKC + kk;
+        const float* pb = hw + 16 * wave * kFiltNB + i - kk + kl - 1;
+        for (int q = 0; q < kl; q += 4) {
+            const float a = pa[q];
+#pragma unroll
+            for (int u = 0; u < kFiltNB; u++)
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, pb[16 * u - q], acc[u], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kFiltNB; u++) {
+        const int col = c0 + 16 * (wave * kFiltNB + u) + i;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = r0 + 4 * kk + r;
+            if (row < rows && col < ndet) filt[(size_t)row * ndet + col] = acc[u][r];
+        }
     }
 }
 
@@ -262,20 +274,20 @@ hipError_t launch_fbp_project(const float* img, float* padded, float* sino, cons
     const int np = g.n + 2 * kProjPad;
     const size_t total = (size_t)batch * np * np;
     hipLaunchKernelGGL(k_fbp_pad, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, padded, g.n, np, total);
+    const unsigned groups = (unsigned)((g.ndet + 31) / 32);  // 4 waves x 8 bins per workgroup
     if (n_bins > 0)
-        hipLaunchKernelGGL(k_fbp_project_bins, dim3((g.ndet + 255) / 256, n_bins, batch), dim3(256), 0, s, padded,
-                           sino, cs, ang_bins, g);
+        hipLaunchKernelGGL(k_fbp_project, dim3(groups, n_bins, batch), dim3(256), 0, s, padded, sino, cs, ang_bins, g);
     if (n_rays > 0)
-        hipLaunchKernelGGL(k_fbp_project_rays, dim3((g.ndet + 3) / 4, n_rays, batch), dim3(256), 0, s, padded, sino,
-                           cs, ang_rays, g);
+        hipLaunchKernelGGL(k_fbp_project, dim3(groups, n_rays, batch), dim3(256), 0, s, padded, sino, cs, ang_rays, g);
     return hipGetLastError();
 }
 
 hipError_t launch_fbp_filter(const float* sino, float* filt, const float* taps, const FbpGeom& g, int batch,
                              hipStream_t s) {
-    const size_t lds = sizeof(float) * (size_t)(3 * g.ndet - 1);
-    hipLaunchKernelGGL(k_fbp_filter, dim3((unsigned)((size_t)batch * g.nang)), dim3(256), lds, s, sino, filt, taps,
-                       g.ndet);
+    const int rows = batch * g.nang;
+    const int kpad = (g.ndet + 3) & ~3;  // K in steps of 4
+    const dim3 grid((unsigned)((rows + 15) / 16), (unsigned)((g.ndet + 16 * kFiltTiles - 1) / (16 * kFiltTiles)));
+    hipLaunchKernelGGL(k_fbp_filter_mfma, grid, dim3(256), 0, s, sino, filt, taps, g.ndet, rows, kpad);
     return hipGetLastError();
 }
 

@@ -3,11 +3,12 @@
 // CNN on the reconstruction:
 //
 //   r0 = BP(filter_h(p))                      (fp32, n x n)
-//   a1 = relu(conv3x3(r0;  1->16) + b1)       (bf16, NHWC)
-//   a2 = relu(conv3x3(a1; 16->16) + b2)
-//   a3 = relu(conv3x3(a2; 16->16) + b3)
-//   out = r0 + conv3x3(a3; 16->1) + b4        (fp32)
+//   a1 = relu(conv3x3(r0; 1->C) + b1)         (bf16, NHWC)
+//   a2 = relu(conv3x3(a1; C->C) + b2)
+//   a3 = relu(conv3x3(a2; C->C) + b3)
+//   out = r0 + conv3x3(a3; C->1) + b4         (fp32)
 //
+// with C = 16 or 32 hidden channels (every CNN kernel is a template on C).
 // Weights are kept in fp32 and rounded to bf16 where they are read
 // (bf16 weights); activations a1..a3 are stored in bf16; every sum
 // accumulates in fp32.  Convolutions use zero padding; tap t = 3*ky + kx
@@ -41,7 +42,6 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }  // value as bf16
 
-constexpr int C = kLC;  // hidden channels
 constexpr int T = 16;  // tile side
 
 // ---------------------------------------------------------------------------
@@ -51,16 +51,18 @@ constexpr int T = 16;  // tile side
 // W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout, and W1 as
 // bf16 [co][16] with k = 4*ky + kx (kx = 3 and k >= 12 zero): the
 // matrix-core operands.
+template <int C>
 __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb2, uint16_t* __restrict__ wb3,
                                uint16_t* __restrict__ w1b) {
+    constexpr LearnedLayout L = learned_layout(C);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 9 * C * C) {
-        wb2[i] = f2bf(prm[kW2 + i]);
-        wb3[i] = f2bf(prm[kW3 + i]);
+        wb2[i] = f2bf(prm[L.W2 + i]);
+        wb3[i] = f2bf(prm[L.W3 + i]);
     }
     if (i < 16 * C) {
         const int co = i / 16, k = i % 16, ky = k / 4, kx = k % 4;
-        w1b[i] = (k < 12 && kx < 3) ? f2bf(prm[kW1 + co * 9 + 3 * ky + kx]) : (uint16_t)0;
+        w1b[i] = (k < 12 && kx < 3) ? f2bf(prm[L.W1 + co * 9 + 3 * ky + kx]) : (uint16_t)0;
     }
 }
 
@@ -74,25 +76,26 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // training forward and the inference forward agree bit for bit.
 //
 // Matrix cores: v_mfma_f32_16x16x32_bf16 computes D[co][px] += sum_k
-// W[co][k] X[k][px] for a group of 16 pixels (fp32 accumulate, the bias as
-// the initial accumulator).  Lane layout: A[i][k] at lane i + 16*(k/8),
-// element k%8; B[k][j] at lane j + 16*(k/8), element k%8; D[i][j] at lane
-// j + 16*(i/4), register i%4.
-//   conv 1 -> 16: k = 4*ky + kx over kx = 0..3 (the kx = 3 weight is zero)
-//   for the three rows, then zeros: one MFMA per group; the operand is
-//   read as bf16 pairs (x, x+1) and (x+2, x+3) from two copies of the r0
-//   tile packed at even and at odd columns;
-//   conv 16 -> 16: k = (tap, channel) over the tap pairs (0,1), (2,3),
-//   (4,5), (6,7), (8,-): five MFMAs per group, taps in order.
-//   conv 16 -> 1: v_dot2c_f32_bf16 on channel pairs, channels 0-7 and 8-15
-//   in two partial sums (taps in order, pairs in order), then (s0+s1) + b4.
-// Activation tiles are bf16 NHWC in LDS at a 32-byte pixel pitch (no pad:
-// a 48-byte pitch avoids bank conflicts but costs two workgroups per CU,
-// and measured slower).
+// W[co][k] X[k][px] for a group of 16 pixels and a block of 16 output
+// channels (fp32 accumulate, the bias as the initial accumulator); C = 32
+// runs two channel blocks on the same pixel operand.  Lane layout: A[i][k]
+// at lane i + 16*(k/8), element k%8; B[k][j] at lane j + 16*(k/8), element
+// k%8; D[i][j] at lane j + 16*(i/4), register i%4.
+//   conv 1 -> C: k = 4*ky + kx over kx = 0..3 (the kx = 3 weight is zero)
+//   for the three rows, then zeros: one MFMA per group and channel block;
+//   the operand is read as bf16 pairs (x, x+1) and (x+2, x+3) from two
+//   copies of the r0 tile packed at even and at odd columns;
+//   conv C -> C: K blocks of 32 = (32/C) taps x C channels, in tap order:
+//   the tap pairs (0,1), (2,3), (4,5), (6,7), (8,-) for C = 16, the nine
+//   taps for C = 32.
+//   conv C -> 1: v_dot2c_f32_bf16 on channel pairs, channels [0, C/2) and
+//   [C/2, C) in two partial sums (taps in order, pairs in order), then
+//   (s0+s1) + b4.
+// Activation tiles are bf16 NHWC in LDS at a pitch of C values per pixel
+// (no pad: a 48-byte pitch at C = 16 avoids bank conflicts but costs two
+// workgroups per CU, and measured slower).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-static_assert(C == 16, "the fused forward is written for 16 hidden channels");
-constexpr int kP = 16;           // LDS pixel pitch in bf16
 constexpr int kFT = 16;          // output tile side
 constexpr int kF0 = kFT + 8;     // r0 region side
 constexpr int kF1 = kFT + 6;     // a1
@@ -101,12 +104,21 @@ constexpr int kF3 = kFT + 2;     // a3
 constexpr int kFWaves = 4;
 constexpr int kS0 = 12;          // bf16 pairs per row of the packed r0 tile
 
+template <int C>
+struct MidK {
+    static_assert(C == 16 || C == 32, "16 or 32 hidden channels");
+    static constexpr int kTapsPer = 32 / C;                          // taps per K block
+    static constexpr int kChunks = C / 8;                            // 8-channel chunks per tap
+    static constexpr int kBlocks = (9 + kTapsPer - 1) / kTapsPer;   // K blocks per layer (5 or 9)
+    static constexpr int kHalves = C / 16;                           // 16-channel output blocks
+};
+
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
 }
 
-// B operand of conv 1 -> 16 for the a1-region pixel (y, x): k = 8*kb ..
+// B operand of conv 1 -> C for the a1-region pixel (y, x): k = 8*kb ..
 // 8*kb+7 with k = 4*ky + kx, i.e. the pairs (x, x+1), (x+2, x+3) of tile
 // rows y, y+1 (kb = 0) or y+2 (kb = 1); `se`/`so` hold the pairs starting at
 // even / odd columns.
@@ -141,77 +153,104 @@ __device__ __forceinline__ void store4(uint16_t* dst, const f32x4& acc, bool kee
     *(uint2*)dst = v;
 }
 
-// A operands of a conv 16 -> 16 layer: lane (co = l%16, kb = l/16) holds
-// W[co][2tp + kb/2][8(kb%2) .. 8(kb%2)+7] for tap pair tp (zero for tap 9).
-__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[5], const uint16_t* __restrict__ wb, int co, int kb) {
+// A operands of a conv C -> C layer: lane (co = l%16, kb = l/16) of channel
+// block h holds W[16h + co][tap][ch .. ch+7] for K block tp, with
+// tap = (32/C)*tp + kb/(C/8) and ch = 8*(kb % (C/8)) (zero for tap 9).
+template <int C>
+__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
+                                                 const uint16_t* __restrict__ wb, int co, int kb) {
+    using K = MidK<C>;
 #pragma unroll
-    for (int tp = 0; tp < 5; tp++) {
-        const int tap = 2 * tp + (kb >> 1);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (tap < 9) v = *(const uint4*)(wb + (co * 9 + tap) * C + 8 * (kb & 1));
-        w[tp] = as_bf16x8(v);
-    }
+    for (int h = 0; h < K::kHalves; h++)
+#pragma unroll
+        for (int tp = 0; tp < K::kBlocks; tp++) {
+            const int tap = K::kTapsPer * tp + kb / K::kChunks;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tap < 9) v = *(const uint4*)(wb + ((16 * h + co) * 9 + tap) * C + 8 * (kb % K::kChunks));
+            w[h][tp] = as_bf16x8(v);
+        }
 }
 
-// One conv 16 -> 16 layer from the LDS region `in` (side wi) to `out` (side
+// One conv C -> C layer from the LDS region `in` (side wi) to `out` (side
 // wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
-// groups g and g + kFWaves together (two independent accumulators).  Lanes
-// of the absent tap 9 read tap 8 again; their weights are zero.
-template <bool kEdge>
+// groups g and g + kFWaves together (two independent accumulators per
+// channel block).  Lanes of an absent tap 9 read tap 8 again; their weights
+// are zero.
+template <int C, bool kEdge>
 __device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
-                                          const bf16x8 (&w)[5], const f32x4 bias, int wave, int j16, int kb) {
+                                          const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
+                                          const f32x4 (&bias)[MidK<C>::kHalves], int wave, int j16, int kb) {
+    using K = MidK<C>;
     const int wo = wi - 2, count = wo * wo;
-    int toff[5];
+    int toff[K::kBlocks];
 #pragma unroll
-    for (int tp = 0; tp < 5; tp++) {
-        const int tap = min(2 * tp + (kb >> 1), 8);
-        toff[tp] = ((tap / 3) * wi + tap % 3) * kP + 8 * (kb & 1);
+    for (int tp = 0; tp < K::kBlocks; tp++) {
+        const int tap = min(K::kTapsPer * tp + kb / K::kChunks, 8);
+        toff[tp] = ((tap / 3) * wi + tap % 3) * C + 8 * (kb % K::kChunks);
     }
     for (int g0 = wave; g0 * 16 < count; g0 += 2 * kFWaves) {
         const int g1 = g0 + kFWaves;
         const int p0 = g0 * 16 + j16, p1 = g1 * 16 + j16;
         const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
         const int y0 = q0 / wo, x0 = q0 % wo, y1 = q1 / wo, x1 = q1 % wo;
-        const uint16_t* b0 = in + (y0 * wi + x0) * kP;
-        f32x4 acc0 = bias;
-        if (g1 * 16 < count) {  // two groups (wave-uniform branch)
-            const uint16_t* b1 = in + (y1 * wi + x1) * kP;
-            f32x4 acc1 = bias;
+        const uint16_t* b0 = in + (y0 * wi + x0) * C;
+        f32x4 acc0[K::kHalves];
 #pragma unroll
-            for (int tp = 0; tp < 5; tp++) {
+        for (int h = 0; h < K::kHalves; h++) acc0[h] = bias[h];
+        if (g1 * 16 < count) {  // two groups (wave-uniform branch)
+            const uint16_t* b1 = in + (y1 * wi + x1) * C;
+            f32x4 acc1[K::kHalves];
+#pragma unroll
+            for (int h = 0; h < K::kHalves; h++) acc1[h] = bias[h];
+#pragma unroll
+            for (int tp = 0; tp < K::kBlocks; tp++) {
                 const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + toff[tp]));
                 const bf16x8 v1 = as_bf16x8(*(const uint4*)(b1 + toff[tp]));
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], v0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], v1, acc1, 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < K::kHalves; h++) {
+                    acc0[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v0, acc0[h], 0, 0, 0);
+                    acc1[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v1, acc1[h], 0, 0, 0);
+                }
             }
             const int iy = oy + y1, ix = ox + x1;
-            if (p1 < count) store4(out + p1 * kP + 4 * kb, acc1, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
+            const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
+            if (p1 < count)
+#pragma unroll
+                for (int h = 0; h < K::kHalves; h++) store4(out + p1 * C + 16 * h + 4 * kb, acc1[h], keep);
         } else {
 #pragma unroll
-            for (int tp = 0; tp < 5; tp++)
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tp], as_bf16x8(*(const uint4*)(b0 + toff[tp])), acc0,
-                                                               0, 0, 0);
+            for (int tp = 0; tp < K::kBlocks; tp++) {
+                const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + toff[tp]));
+#pragma unroll
+                for (int h = 0; h < K::kHalves; h++)
+                    acc0[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v0, acc0[h], 0, 0, 0);
+            }
         }
         const int iy = oy + y0, ix = ox + x0;
-        if (p0 < count) store4(out + p0 * kP + 4 * kb, acc0, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
+        const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
+        if (p0 < count)
+#pragma unroll
+            for (int h = 0; h < K::kHalves; h++) store4(out + p0 * C + 16 * h + 4 * kb, acc0[h], keep);
     }
 }
 
 // Copy the centre 16 x 16 pixels of an LDS region (side wi, centre offset
 // `off`) to the NHWC image `dst` (training: the backward kernels read it).
+template <int C>
 __device__ __forceinline__ void store_centre(const uint16_t* reg, int wi, int off, uint16_t* __restrict__ dst,
                                              int i0, int j0, int n) {
-    for (int t = threadIdx.x; t < kFT * kFT * 2; t += 64 * kFWaves) {
-        const int px = t >> 1, h = t & 1;
+    constexpr int kV = C / 8;  // 16-byte vectors per pixel
+    for (int t = threadIdx.x; t < kFT * kFT * kV; t += 64 * kFWaves) {
+        const int px = t / kV, h = t % kV;
         const int y = px / kFT, x = px % kFT;
-        const uint4 v = *(const uint4*)(reg + ((y + off) * wi + x + off) * kP + 8 * h);
+        const uint4 v = *(const uint4*)(reg + ((y + off) * wi + x + off) * C + 8 * h);
         *(uint4*)(dst + ((size_t)(i0 + y) * n + j0 + x) * C + 8 * h) = v;
     }
 }
 
 // One output tile.  kEdge: some region pixel lies outside the image (zero
 // padding to apply); interior tiles skip the bounds tests.
-template <bool kTrain, bool kEdge>
+template <int C, bool kTrain, bool kEdge>
 __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const float* __restrict__ prm,
                                          const uint16_t* __restrict__ w1b, const uint16_t* __restrict__ wb2,
                                          const uint16_t* __restrict__ wb3, float* __restrict__ out, int n,
@@ -220,21 +259,28 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
                                          float* __restrict__ dout, float inv_count, double* __restrict__ loss,
                                          uint16_t* sa, uint16_t* sb, uint32_t* s0e, uint32_t* s0o, uint32_t* w4,
                                          float* red) {
+    using K = MidK<C>;
+    constexpr int H = K::kHalves;
+    constexpr LearnedLayout L = learned_layout(C);
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j16 = lane & 15, kb = lane >> 4;  // A row / B column, and k block
     // prologue loads in flight together: W2 operands, W1, biases, W4, r0 tile
-    bf16x8 w[5];
-    load_mid_weights(w, wb2, j16, kb);
-    const bf16x8 w1 = as_bf16x8(kb < 2 ? *(const uint4*)(w1b + j16 * 16 + 8 * kb) : make_uint4(0, 0, 0, 0));
-    f32x4 bias1, bias2, bias3;
+    bf16x8 w[H][K::kBlocks];
+    load_mid_weights<C>(w, wb2, j16, kb);
+    bf16x8 w1[H];
+    f32x4 bias1[H], bias2[H], bias3[H];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        bias1[r] = prm[kB1 + 4 * kb + r];
-        bias2[r] = prm[kB2 + 4 * kb + r];
-        bias3[r] = prm[kB3 + 4 * kb + r];
+    for (int h = 0; h < H; h++) {
+        w1[h] = as_bf16x8(kb < 2 ? *(const uint4*)(w1b + (16 * h + j16) * 16 + 8 * kb) : make_uint4(0, 0, 0, 0));
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            bias1[h][r] = prm[L.B1 + 16 * h + 4 * kb + r];
+            bias2[h][r] = prm[L.B2 + 16 * h + 4 * kb + r];
+            bias3[h][r] = prm[L.B3 + 16 * h + 4 * kb + r];
+        }
     }
     {   // all of this thread's r0 loads in flight together
         constexpr int kIt = (kF0 * kS0 + 64 * kFWaves - 1) / (64 * kFWaves);
@@ -261,7 +307,8 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
             }
         }
     }
-    for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves) w4[t] = pack_bf16(prm[kW4 + 2 * t], prm[kW4 + 2 * t + 1]);
+    for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves)
+        w4[t] = pack_bf16(prm[L.W4 + 2 * t], prm[L.W4 + 2 * t + 1]);
     __syncthreads();
     {   // a1 on the 22 x 22 region at (i0-3, j0-3)
         const int count = kF1 * kF1;
@@ -271,49 +318,65 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
             const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
             const int y0 = q0 / kF1, x0 = q0 % kF1, y1 = q1 / kF1, x1 = q1 % kF1;
             const bf16x8 v0 = in_operand(s0e, s0o, y0, x0, kb), v1 = in_operand(s0e, s0o, y1, x1, kb);
-            const f32x4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, v0, bias1, 0, 0, 0);
-            const f32x4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, v1, bias1, 0, 0, 0);
             int iy = i0 - 3 + y0, ix = j0 - 3 + x0;
-            if (p0 < count) store4(sa + p0 * kP + 4 * kb, acc0, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
+            const bool keep0 = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
             iy = i0 - 3 + y1;
             ix = j0 - 3 + x1;
-            if (p1 < count) store4(sa + p1 * kP + 4 * kb, acc1, !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n));
+            const bool keep1 = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+                const f32x4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[h], v0, bias1[h], 0, 0, 0);
+                const f32x4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[h], v1, bias1[h], 0, 0, 0);
+                if (p0 < count) store4(sa + p0 * C + 16 * h + 4 * kb, acc0, keep0);
+                if (p1 < count) store4(sa + p1 * C + 16 * h + 4 * kb, acc1, keep1);
+            }
         }
     }
     __syncthreads();
     const size_t img_off = (size_t)b * n * n * C;
-    if (kTrain) store_centre(sa, kF1, 3, a1g + img_off, i0, j0, n);
-    mid_layer<kEdge>(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
-    load_mid_weights(w, wb3, j16, kb);  // in flight while the other waves finish
+    if (kTrain) store_centre<C>(sa, kF1, 3, a1g + img_off, i0, j0, n);
+    mid_layer<C, kEdge>(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
+    load_mid_weights<C>(w, wb3, j16, kb);  // in flight while the other waves finish
     __syncthreads();
-    if (kTrain) store_centre(sb, kF2, 2, a2g + img_off, i0, j0, n);
-    mid_layer<kEdge>(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
+    if (kTrain) store_centre<C>(sb, kF2, 2, a2g + img_off, i0, j0, n);
+    mid_layer<C, kEdge>(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
-    if (kTrain) store_centre(sa, kF3, 1, a3g + img_off, i0, j0, n);
-    // conv 16 -> 1 + bias + residual: one thread per horizontal pixel pair and
-    // channel half.  The pair's 3 x 4 neighbourhood is read once (12 LDS reads
-    // instead of 2 x 9); each pixel still sums its taps 0..8 in order.
-    float dsum = 0.0f;  // training: this thread's squared errors
+    if (kTrain) store_centre<C>(sa, kF3, 1, a3g + img_off, i0, j0, n);
+    // conv C -> 1 + bias + residual: one thread per horizontal pixel pair and
+    // channel half.  The pair's 3 x 4 neighbourhood is read once (12 reads
+    // of each 16-byte vector instead of 2 x 9); each pixel still sums its
+    // taps 0..8 in order.
+    constexpr int kQ = C / 16;  // 16-byte vectors per pixel and channel half
+    float dsum = 0.0f;          // training: this thread's squared errors
     for (int t = threadIdx.x; t < kFT * kFT; t += 64 * kFWaves) {
         const int pr = t >> 1, part = t & 1;
         const int y = pr / (kFT / 2), x = 2 * (pr % (kFT / 2));
         float acc0 = 0.0f, acc1 = 0.0f;
 #pragma unroll
         for (int ty = 0; ty < 3; ty++) {
-            uint4 c[4];
+            uint4 c[4][kQ];
 #pragma unroll
-            for (int cx = 0; cx < 4; cx++) c[cx] = *(const uint4*)(sa + ((y + ty) * kF3 + x + cx) * kP + 8 * part);
+            for (int cx = 0; cx < 4; cx++)
+#pragma unroll
+                for (int q = 0; q < kQ; q++)
+                    c[cx][q] = *(const uint4*)(sa + ((y + ty) * kF3 + x + cx) * C + (C / 2) * part + 8 * q);
 #pragma unroll
             for (int tx = 0; tx < 3; tx++) {
-                const uint32_t* wp = w4 + (3 * ty + tx) * (C / 2) + 4 * part;
-                acc0 = dot2_bf16(wp[0], c[tx].x, acc0);
-                acc0 = dot2_bf16(wp[1], c[tx].y, acc0);
-                acc0 = dot2_bf16(wp[2], c[tx].z, acc0);
-                acc0 = dot2_bf16(wp[3], c[tx].w, acc0);
-                acc1 = dot2_bf16(wp[0], c[tx + 1].x, acc1);
-                acc1 = dot2_bf16(wp[1], c[tx + 1].y, acc1);
-                acc1 = dot2_bf16(wp[2], c[tx + 1].z, acc1);
-                acc1 = dot2_bf16(wp[3], c[tx + 1].w, acc1);
+                const uint32_t* wp = w4 + (3 * ty + tx) * (C / 2) + (C / 4) * part;
+#pragma unroll
+                for (int q = 0; q < kQ; q++) {
+                    acc0 = dot2_bf16(wp[4 * q + 0], c[tx][q].x, acc0);
+                    acc0 = dot2_bf16(wp[4 * q + 1], c[tx][q].y, acc0);
+                    acc0 = dot2_bf16(wp[4 * q + 2], c[tx][q].z, acc0);
+                    acc0 = dot2_bf16(wp[4 * q + 3], c[tx][q].w, acc0);
+                }
+#pragma unroll
+                for (int q = 0; q < kQ; q++) {
+                    acc1 = dot2_bf16(wp[4 * q + 0], c[tx + 1][q].x, acc1);
+                    acc1 = dot2_bf16(wp[4 * q + 1], c[tx + 1][q].y, acc1);
+                    acc1 = dot2_bf16(wp[4 * q + 2], c[tx + 1][q].z, acc1);
+                    acc1 = dot2_bf16(wp[4 * q + 3], c[tx + 1][q].w, acc1);
+                }
             }
         }
         // the other half's sums; the part-0 thread finishes pixel x, part 1 pixel x + 1
@@ -322,7 +385,7 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
         const float s1v = part == 0 ? o0 : acc1;  // channel half 1
         const int xx = x + part;
         const size_t p = ((size_t)b * n + i0 + y) * n + j0 + xx;
-        const float o = img[(size_t)(i0 + y) * n + j0 + xx] + ((s0v + s1v) + prm[kB4]);
+        const float o = img[(size_t)(i0 + y) * n + j0 + xx] + ((s0v + s1v) + prm[L.B4]);
         out[p] = o;
         if (kTrain && target) {
             const float d = o - target[p];
@@ -342,7 +405,7 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
 }
 
 
-template <bool kTrain>
+template <int C, bool kTrain>
 __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __restrict__ r0,
                                                                 const float* __restrict__ prm,
                                                                 const uint16_t* __restrict__ w1b,
@@ -354,19 +417,19 @@ __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __r
                                                                 const float* __restrict__ target,
                                                                 float* __restrict__ dout, float inv_count,
                                                                 double* __restrict__ loss) {
-    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * kP];  // a1, then a3
-    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * kP];  // a2
+    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * C];  // a1, then a3
+    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * C];  // a2
     __shared__ uint32_t s0e[kF0 * kS0], s0o[kF0 * kS0];   // r0 as bf16 pairs at even / odd columns
     __shared__ uint32_t w4[9 * C / 2];                    // W4 as bf16 pairs
     __shared__ float red[64 * kFWaves];
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const bool interior = i0 >= 4 && i0 + kFT + 4 <= n && j0 >= 4 && j0 + kFT + 4 <= n;  // uniform
     if (interior)
-        cnn_tile<kTrain, false>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa, sb,
-                                s0e, s0o, w4, red);
+        cnn_tile<C, kTrain, false>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa,
+                                   sb, s0e, s0o, w4, red);
     else
-        cnn_tile<kTrain, true>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa, sb,
-                               s0e, s0o, w4, red);
+        cnn_tile<C, kTrain, true>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa,
+                                  sb, s0e, s0o, w4, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -375,13 +438,14 @@ __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __r
 // ---------------------------------------------------------------------------
 
 // dA3[q][ci] = sum_t W4[t][ci] * dout[q - off(t)], times (a3 > 0).
+template <int C>
 __global__ __launch_bounds__(256) void k_conv_out_bwd(const float* __restrict__ dout, const uint16_t* __restrict__ a3,
                                                       const float* __restrict__ prm, float* __restrict__ dz3, int n) {
     __shared__ float w[9 * C];
     __shared__ float ds[(T + 2) * (T + 2)];
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    for (int t = threadIdx.x; t < 9 * C; t += 256) w[t] = rbf(prm[kW4 + t]);
+    for (int t = threadIdx.x; t < 9 * C; t += 256) w[t] = rbf(prm[learned_layout(C).W4 + t]);
     const float* db = dout + (size_t)b * n * n;
     for (int t = threadIdx.x; t < (T + 2) * (T + 2); t += 256) {
         const int ii = i0 + t / (T + 2) - 1, jj = j0 + t % (T + 2) - 1;
@@ -404,6 +468,7 @@ __global__ __launch_bounds__(256) void k_conv_out_bwd(const float* __restrict__ 
 // dX[q][ci] = sum_{t,co} W[co][t][ci] * dZ[q - off(t)][co], times (x > 0).
 // 16 x 8 pixel tiles so that the fp32 dZ halo and the weights fit in LDS.
 constexpr int TY = 8;
+template <int C>
 __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ dz, const uint16_t* __restrict__ x,
                                                       const float* __restrict__ prm, int woff,
                                                       float* __restrict__ dzp, int n) {
@@ -442,13 +507,14 @@ __global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ 
 }
 
 // dr0[q] = dout[q] + sum_{t,co} W1[co][t] * dZ1[q - off(t)][co].
+template <int C>
 __global__ __launch_bounds__(128) void k_conv_in_bwd(const float* __restrict__ dz1, const float* __restrict__ dout,
                                                      const float* __restrict__ prm, float* __restrict__ dr0, int n) {
     __shared__ float w[C * 9];
     __shared__ float ds[(TY + 2) * (T + 2) * (C + 1)];  // pixel pitch C+1: distinct banks per lane
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * T;
-    for (int t = threadIdx.x; t < C * 9; t += 128) w[t] = rbf(prm[kW1 + t]);
+    for (int t = threadIdx.x; t < C * 9; t += 128) w[t] = rbf(prm[learned_layout(C).W1 + t]);
     const float* db = dz1 + (size_t)b * n * n * C;
     for (int t = threadIdx.x; t < (TY + 2) * (T + 2) * C; t += 128) {
         const int pix = t / C, c = t % C;
@@ -476,7 +542,7 @@ __global__ __launch_bounds__(128) void k_conv_in_bwd(const float* __restrict__ d
 // dZ: fp32 NHWC with zc channels.  16 x 8 pixel tiles; a workgroup sums its
 // tile over the slices b = blockIdx.z, blockIdx.z + gridDim.z, ... in
 // registers and adds the partial sums into grad[] with atomics once.
-constexpr int kWgPer = 9 * C * C / 256;  // weights per thread (largest layer)
+template <int C>
 __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dz, int zc, const uint16_t* __restrict__ xh,
                                                const float* __restrict__ xf, int xc, float* __restrict__ grad, int woff,
                                                int boff, int wlayout_tc, int n, int batch) {
@@ -484,6 +550,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dz, int
     float* zs = sm;                        // TY*T*zc
     float* xs = sm + TY * T * zc;          // (TY+2)(T+2)*xc
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * T;
+    constexpr int kWgPer = 9 * C * C / 256;  // weights per thread (largest layer)
     const int nw = zc * 9 * xc;
     float acc[kWgPer];
 #pragma unroll
@@ -695,54 +762,72 @@ __global__ void k_noise(float* __restrict__ sino, size_t count, float i0, float 
 
 #define SKE_GRID(n, b) dim3((n) / T, (n) / T, (b))
 
-hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_weights, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb2, wb3, w1b);
+template <int C>
+hipError_t pack_weights_t(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_weights<C>, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb2, wb3, w1b);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
+    return C == 32 ? pack_weights_t<32>(prm, wb2, wb3, w1b, s) : pack_weights_t<16>(prm, wb2, wb3, w1b, s);
+}
+
+template <int C, bool kTrain>
+hipError_t fused_t(const LearnedBufs& B, const float* prm, int n, int batch, const float* target, float* dout,
+                   double* loss, hipStream_t s) {
+    const float inv = 1.0f / ((float)n * n * batch);
+    hipLaunchKernelGGL((k_cnn_fused_fwd<C, kTrain>), dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0,
+                       prm, B.w1b, B.wb2, B.wb3, B.out, n, kTrain ? B.a1 : nullptr, kTrain ? B.a2 : nullptr,
+                       kTrain ? B.a3 : nullptr, target, dout, inv, loss);
     return hipGetLastError();
 }
 
 // Inference: the fused forward, no activations stored (n a multiple of 16).
 hipError_t launch_cnn_infer(const LearnedBufs& B, const float* prm, int n, int batch, hipStream_t s) {
-    hipLaunchKernelGGL(k_cnn_fused_fwd<false>, dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0, prm,
-                       B.w1b, B.wb2, B.wb3, B.out, n, (uint16_t*)nullptr, (uint16_t*)nullptr, (uint16_t*)nullptr,
-                       (const float*)nullptr, (float*)nullptr, 0.0f, (double*)nullptr);
-    return hipGetLastError();
+    return B.C == 32 ? fused_t<32, false>(B, prm, n, batch, nullptr, nullptr, nullptr, s)
+                     : fused_t<16, false>(B, prm, n, batch, nullptr, nullptr, nullptr, s);
 }
 
 // Training forward: the same fused kernel, storing a1..a3 (centre tiles),
 // out, and with a target the loss gradient and the summed squared error.
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s) {
-    const float inv = 1.0f / ((float)n * n * batch);
-    hipLaunchKernelGGL(k_cnn_fused_fwd<true>, dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0, prm,
-                       B.w1b, B.wb2, B.wb3, B.out, n, B.a1, B.a2, B.a3, target, dout, inv, loss);
-    return hipGetLastError();
+    return B.C == 32 ? fused_t<32, true>(B, prm, n, batch, target, dout, loss, s)
+                     : fused_t<16, true>(B, prm, n, batch, target, dout, loss, s);
 }
 
-hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const float* dout, float* grad, int n,
-                               int batch, hipStream_t s) {
+template <int C>
+hipError_t backward_t(const LearnedBufs& B, const float* prm, const float* dout, float* grad, int n, int batch,
+                      hipStream_t s) {
+    constexpr LearnedLayout L = learned_layout(C);
     // activation gradients
-    hipLaunchKernelGGL(k_conv_out_bwd, SKE_GRID(n, batch), dim3(256), 0, s, dout, B.a3, prm, B.dz3, n);
+    hipLaunchKernelGGL(k_conv_out_bwd<C>, SKE_GRID(n, batch), dim3(256), 0, s, dout, B.a3, prm, B.dz3, n);
     const size_t lmid = sizeof(float) * (9 * C * C + (TY + 2) * (T + 2) * (C + 1));
     dim3 g8(n / T, n / TY, batch);
-    hipLaunchKernelGGL(k_conv_mid_bwd, g8, dim3(128), lmid, s, B.dz3, B.a2, prm, kW3, B.dz2, n);
-    hipLaunchKernelGGL(k_conv_mid_bwd, g8, dim3(128), lmid, s, B.dz2, B.a1, prm, kW2, B.dz1, n);
-    hipLaunchKernelGGL(k_conv_in_bwd, g8, dim3(128), 0, s, B.dz1, dout, prm, B.dr0, n);
+    hipLaunchKernelGGL(k_conv_mid_bwd<C>, g8, dim3(128), lmid, s, B.dz3, B.a2, prm, L.W3, B.dz2, n);
+    hipLaunchKernelGGL(k_conv_mid_bwd<C>, g8, dim3(128), lmid, s, B.dz2, B.a1, prm, L.W2, B.dz1, n);
+    hipLaunchKernelGGL(k_conv_in_bwd<C>, g8, dim3(128), 0, s, B.dz1, dout, prm, B.dr0, n);
     // weight gradients: about 2048 workgroups, each summing over a share of
     // the slices
     const int tiles = (n / T) * (n / TY);
     const dim3 gw(n / T, n / TY, (unsigned)std::max(1, std::min(batch, 2048 / tiles)));
     const size_t lw4 = sizeof(float) * (TY * T * 1 + (TY + 2) * (T + 2) * C);
-    hipLaunchKernelGGL(k_wgrad, gw, dim3(256), lw4, s, dout, 1, B.a3, (const float*)nullptr, C, grad, kW4, kB4, 1, n,
-                       batch);
-    const size_t lwm = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * C);
-    hipLaunchKernelGGL(k_wgrad, gw, dim3(256), lwm, s, B.dz3, C, B.a2, (const float*)nullptr, C, grad, kW3, kB3, 0, n,
-                       batch);
-    hipLaunchKernelGGL(k_wgrad, gw, dim3(256), lwm, s, B.dz2, C, B.a1, (const float*)nullptr, C, grad, kW2, kB2, 0, n,
-                       batch);
-    const size_t lw1 = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * 1);
-    hipLaunchKernelGGL(k_wgrad, gw, dim3(256), lw1, s, B.dz1, C, (const uint16_t*)nullptr, B.r0, 1, grad, kW1, kB1, 0,
+    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lw4, s, dout, 1, B.a3, (const float*)nullptr, C, grad, L.W4, L.B4, 1,
                        n, batch);
+    const size_t lwm = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * C);
+    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lwm, s, B.dz3, C, B.a2, (const float*)nullptr, C, grad, L.W3, L.B3, 0,
+                       n, batch);
+    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lwm, s, B.dz2, C, B.a1, (const float*)nullptr, C, grad, L.W2, L.B2, 0,
+                       n, batch);
+    const size_t lw1 = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * 1);
+    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lw1, s, B.dz1, C, (const uint16_t*)nullptr, B.r0, 1, grad, L.W1,
+                       L.B1, 0, n, batch);
     return hipGetLastError();
+}
+
+hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const float* dout, float* grad, int n,
+                               int batch, hipStream_t s) {
+    return B.C == 32 ? backward_t<32>(B, prm, dout, grad, n, batch, s) : backward_t<16>(B, prm, dout, grad, n, batch, s);
 }
 
 hipError_t launch_filter_backward(const float* dr0, float* dq, const float* sino, float* dfull, float* dsym,

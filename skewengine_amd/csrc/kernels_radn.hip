@@ -276,6 +276,60 @@ __global__ __launch_bounds__(64) void k_op_lrt(const SecDesc* __restrict__ secs,
     }
 }
 
+// The vectors the reference records at a sample point (pointRadon,
+// kernelRADN.h:117-170, for the cells of samplePoints): x = the 64 windowed
+// sums (from W) times the Gaussian, the Gaussian-weighted row through the
+// cell (column j - 32 + kk of row i, inside the extents of row k = i - 32 + kk
+// as the reference reads them; 0 for k outside [0, H)), y = the filtered
+// real spectrum and Re FFT64(y)/8.  One 64-lane wave per request; direct
+// sums in FP64 (cos/sin(2 pi t/64) from tab).
+__global__ __launch_bounds__(64) void k_lrt_samples(const SecDesc* __restrict__ secs,
+                                                    const SampleReq* __restrict__ req,
+                                                    const uint32_t* __restrict__ fl, const double* __restrict__ W,
+                                                    const float* __restrict__ in4, const int* __restrict__ target,
+                                                    const float* __restrict__ weight, const double* __restrict__ tab,
+                                                    const double* __restrict__ gauss, float* __restrict__ out) {
+    __shared__ double xs[64], ys[64];
+    const SampleReq r = req[blockIdx.x];
+    const SecDesc sd = secs[r.sec];
+    const int kk = threadIdx.x;
+    const double g = gauss[64 * sd.newAngle + kk];
+    const int k = r.i - 32 + kk;
+    double win = 0.0, axis = 0.0;
+    if (k >= 0 && k < sd.H) {
+        win = W[sd.so + (size_t)k * sd.dim_i + r.j];
+        const uint32_t v = fl[sd.rows + k];
+        const int c = r.j - 32 + kk;
+        if (c >= fl_first(v) && c < fl_last(v)) {
+            const float* src = in4 + sd.img + (size_t)sd.type * sd.N;
+            axis = g * (double)skew_cell(src, target[sd.tab + c], weight[sd.tab + c], r.i, c, sd.dim_i, sd.dim_o);
+        }
+    }
+    const double x = win * g;
+    xs[kk] = x;
+    __syncthreads();
+    double re = 0.0;
+    for (int m = 0; m < 64; ++m) re = __builtin_fma(xs[m], tab[2 * ((kk * m) & 63)], re);
+    const double y = kk <= 25 ? tab[160 + kk] * (re * 0.125) : 0.0;  // cutoff 0.4 * 64, sine ramp
+    ys[kk] = y;
+    __syncthreads();
+    double yr = 0.0;
+    for (int m = 0; m < 64; ++m) yr = __builtin_fma(ys[m], tab[2 * ((kk * m) & 63)], yr);
+    float* o = out + (size_t)r.slot * 256;
+    o[kk] = (float)x;
+    o[64 + kk] = (float)axis;
+    o[128 + kk] = (float)y;
+    o[192 + kk] = (float)(yr * 0.125);
+}
+
+hipError_t launch_lrt_samples(const DevPlan& p, const SampleReq* req, int nreq, const float* in4, const double* W,
+                              const double* tab, const double* gauss, float* out, hipStream_t s) {
+    if (nreq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lrt_samples, dim3(nreq), dim3(64), 0, s, p.secs, req, p.fl, W, in4, p.target, p.weight, tab,
+                       gauss, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_lrt(const DevPlan& p, const float* in4, const float* Sbuf, double* W, float* arena,
                       const double* tab, const double* gauss, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;

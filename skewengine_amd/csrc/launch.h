@@ -72,6 +72,11 @@ hipError_t launch_radn_filter(const DevPlan& p, const double* P, float* q, const
 hipError_t launch_lrt(const DevPlan& p, const float* in4, const float* Sbuf, double* W, float* arena,
                       const double* tab, const double* gauss, hipStream_t s);
 
+// The four 64-float vectors of the local transform at `nreq` sample cells
+// (after launch_lrt filled W for the same plan) into out[slot * 256 ...].
+hipError_t launch_lrt_samples(const DevPlan& p, const SampleReq* req, int nreq, const float* in4, const double* W,
+                              const double* tab, const double* gauss, float* out, hipStream_t s);
+
 // q -> skewOutput rows on [first, last) (materialised per-sector path).
 hipError_t launch_radn_smear(const DevPlan& p, const float* q, float* SO, hipStream_t s);
 

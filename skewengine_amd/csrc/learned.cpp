@@ -20,6 +20,8 @@ using namespace ske;
 
 struct ske_learned {
     ske_fbp* f = nullptr;
+    int C = 16;                    // hidden channels (16 or 32)
+    LearnedLayout L = learned_layout(16);
     int nparam = 0;
     int step = 0;                  // Adam steps taken
     bool derived = false;          // full taps and bf16 weights match prm (cleared whenever prm changes)
@@ -31,7 +33,8 @@ struct ske_learned {
     size_t pix(int batch) const { return (size_t)f->g.n * f->g.n * (size_t)batch; }
     LearnedBufs bufs(int batch, bool train) const {
         LearnedBufs B{};
-        const size_t a = pix(f->max_batch) * kLC;
+        const size_t a = pix(f->max_batch) * C;
+        B.C = C;
         B.r0 = r0.p;
         B.a1 = act.p;
         B.a2 = act.p + a;
@@ -44,8 +47,8 @@ struct ske_learned {
             B.dr0 = dr0.p;
         }
         B.wb2 = wb.p;
-        B.wb3 = wb.p + 9 * kLC * kLC;
-        B.w1b = wb.p + 2 * 9 * kLC * kLC;
+        B.wb3 = wb.p + 9 * C * C;
+        B.w1b = wb.p + 2 * 9 * C * C;
         (void)batch;
         return B;
     }
@@ -97,9 +100,10 @@ int forward(ske_learned* l, int batch, bool train) {
     const LearnedBufs B = l->bufs(batch, train);
     if (!l->derived) {  // the parameters changed: expanded taps and bf16 matrix-core weights
         SKE_TRY(f->timer.launch(s, "learned_params", [&] {
-            hipError_t e = launch_taps_expand(l->prm.p + kTaps, l->full.p, g.ndet, s);
+            hipError_t e = launch_taps_expand(l->prm.p + l->L.Taps, l->full.p, g.ndet, s);
             if (e != hipSuccess) return e;
-            return launch_pack_weights(l->prm.p, l->wb.p, l->wb.p + 9 * kLC * kLC, l->wb.p + 2 * 9 * kLC * kLC, s);
+            const int C = l->C;
+            return launch_pack_weights(l->prm.p, C, l->wb.p, l->wb.p + 9 * C * C, l->wb.p + 2 * 9 * C * C, s);
         }));
         l->derived = true;
     }
@@ -122,7 +126,7 @@ int forward(ske_learned* l, int batch, bool train) {
 int ensure_train(ske_learned* l) {
     const int mb = l->f->max_batch;
     const FbpGeom& g = l->f->g;
-    SKE_TRY(l->dz.ensure(3 * l->pix(mb) * kLC));
+    SKE_TRY(l->dz.ensure(3 * l->pix(mb) * l->C));
     SKE_TRY(l->dr0.ensure(l->pix(mb)));
     SKE_TRY(l->dout.ensure(l->pix(mb)));
     SKE_TRY(l->dq.ensure((size_t)mb * g.nang * g.ndet));
@@ -150,7 +154,7 @@ int gradient(ske_learned* l, int batch) {
         if (e != hipSuccess) return e;
         e = launch_fill_f32(l->dfull.p, 2 * (size_t)g.ndet - 1, 0.0f, s);
         if (e != hipSuccess) return e;
-        return launch_filter_backward(l->dr0.p, l->dq.p, f->buf[SKE_FBP_SINO].p, l->dfull.p, l->grad.p + kTaps,
+        return launch_filter_backward(l->dr0.p, l->dq.p, f->buf[SKE_FBP_SINO].p, l->dfull.p, l->grad.p + l->L.Taps,
                                       f->cs.p, g, batch, s);
     });
 }
@@ -168,20 +172,25 @@ int read_loss(ske_learned* l, int batch, double* loss) {
 
 extern "C" {
 
-int ske_learned_create(ske_fbp* f, ske_learned** out) {
+int ske_learned_create(ske_fbp* f, ske_learned** out) { return ske_learned_create_channels(f, 16, out); }
+
+int ske_learned_create_channels(ske_fbp* f, int channels, ske_learned** out) {
     if (!out) return fail(SKE_EINVAL, "out is NULL");
     *out = nullptr;
     if (!f) return fail(SKE_EINVAL, "fbp is NULL");
+    if (!learned_channels_ok(channels)) return fail(SKE_EINVAL, "hidden channels must be 16 or 32");
     if (f->g.n % 32) return fail(SKE_EUNSUPPORTED, "learned FBP needs an image side that is a multiple of 32");
     SKE_HIP(hipSetDevice(f->device));
     std::unique_ptr<ske_learned> l(new ske_learned());
     l->f = f;
-    l->nparam = kTaps + f->g.ndet;
+    l->C = channels;
+    l->L = learned_layout(channels);
+    l->nparam = l->L.Taps + f->g.ndet;
     const size_t pix = l->pix(f->max_batch);
     SKE_TRY(l->prm.ensure(l->nparam));
     SKE_TRY(l->r0.ensure(pix));
-    SKE_TRY(l->act.ensure(3 * pix * kLC));
-    SKE_TRY(l->wb.ensure(2 * 9 * kLC * kLC + 16 * kLC));
+    SKE_TRY(l->act.ensure(3 * pix * channels));
+    SKE_TRY(l->wb.ensure(2 * 9 * channels * channels + 16 * channels));
     SKE_TRY(l->full.ensure(2 * (size_t)f->g.ndet - 1));
     SKE_TRY(ske_learned_init(l.get(), 1));
     *out = l.release();
@@ -202,17 +211,26 @@ int ske_learned_param_count(ske_learned* l, int* count) {
     return SKE_OK;
 }
 
+int ske_learned_layout(ske_learned* l, int* channels, int* taps_offset) {
+    if (!l) return fail(SKE_EINVAL, "learned is NULL");
+    if (channels) *channels = l->C;
+    if (taps_offset) *taps_offset = l->L.Taps;
+    return SKE_OK;
+}
+
 int ske_learned_init(ske_learned* l, unsigned long long seed) {
     if (!l) return fail(SKE_EINVAL, "learned is NULL");
     std::vector<float> p(l->nparam, 0.0f);
     Rng r(seed);
     // He-normal hidden layers, zero biases, zero output layer: out = FBP.
-    for (int i = 0; i < kLC * 9; i++) p[kW1 + i] = (float)(r.normal() * sqrt(2.0 / 9.0));
-    for (int i = 0; i < 9 * kLC * kLC; i++) p[kW2 + i] = (float)(r.normal() * sqrt(2.0 / (9.0 * kLC)));
-    for (int i = 0; i < 9 * kLC * kLC; i++) p[kW3 + i] = (float)(r.normal() * sqrt(2.0 / (9.0 * kLC)));
+    const int C = l->C;
+    const LearnedLayout& L = l->L;
+    for (int i = 0; i < C * 9; i++) p[L.W1 + i] = (float)(r.normal() * sqrt(2.0 / 9.0));
+    for (int i = 0; i < 9 * C * C; i++) p[L.W2 + i] = (float)(r.normal() * sqrt(2.0 / (9.0 * C)));
+    for (int i = 0; i < 9 * C * C; i++) p[L.W3 + i] = (float)(r.normal() * sqrt(2.0 / (9.0 * C)));
     std::vector<float> rl;
     ramlak_taps(l->f->g.ndet, 2.0 / l->f->g.n, rl);
-    for (int k = 0; k < l->f->g.ndet; k++) p[kTaps + k] = rl[l->f->g.ndet - 1 + k];
+    for (int k = 0; k < l->f->g.ndet; k++) p[L.Taps + k] = rl[l->f->g.ndet - 1 + k];
     return ske_learned_set_params(l, p.data());
 }
 
@@ -260,7 +278,7 @@ int ske_learned_train_step(ske_learned* l, int batch, float lr, float lr_taps, d
     SKE_TRY(gradient(l, batch));
     l->step++;
     SKE_TRY(l->f->timer.launch(l->f->stream, "adam", [&] {
-        return launch_adam(l->prm.p, l->grad.p, l->m1.p, l->m2.p, l->nparam, lr, lr_taps, kTaps, l->step,
+        return launch_adam(l->prm.p, l->grad.p, l->m1.p, l->m2.p, l->nparam, lr, lr_taps, l->L.Taps, l->step,
                            l->f->stream);
     }));
     l->derived = false;
@@ -270,7 +288,7 @@ int ske_learned_train_step(ske_learned* l, int batch, float lr, float lr_taps, d
 int ske_learned_download(ske_learned* l, int which, int batch, void* host) {
     SKE_TRY(check(l, batch));
     if (!host) return fail(SKE_EINVAL, "host is NULL");
-    const size_t pix = l->pix(batch), a = l->pix(l->f->max_batch) * kLC;
+    const size_t pix = l->pix(batch), a = l->pix(l->f->max_batch) * l->C;
     const void* src = nullptr;
     size_t bytes = 0;
     switch (which) {
@@ -279,7 +297,7 @@ int ske_learned_download(ske_learned* l, int which, int batch, void* host) {
         case SKE_LEARNED_A2:
         case SKE_LEARNED_A3:
             src = l->act.p + (which - SKE_LEARNED_A1) * a;
-            bytes = pix * kLC * sizeof(uint16_t);
+            bytes = pix * l->C * sizeof(uint16_t);
             break;
         case SKE_LEARNED_DR0:
             if (!l->dr0.p) return fail(SKE_ESTATE, "no gradient computed yet");

@@ -8,24 +8,32 @@
 
 namespace ske {
 
-// Flat fp32 parameter vector: CNN weights and biases, then the symmetric
-// filter taps sym[0..ndet-1] (full filter h[m] = sym[|m|]).
-constexpr int kLC = 16;                 // hidden channels
-constexpr int kW1 = 0;                  // [16][9]       conv 1 -> 16
-constexpr int kB1 = kW1 + kLC * 9;      // [16]
-constexpr int kW2 = kB1 + kLC;          // [16][9][16]   conv 16 -> 16
-constexpr int kB2 = kW2 + kLC * 9 * kLC;
-constexpr int kW3 = kB2 + kLC;          // [16][9][16]   conv 16 -> 16
-constexpr int kB3 = kW3 + kLC * 9 * kLC;
-constexpr int kW4 = kB3 + kLC;          // [9][16]       conv 16 -> 1
-constexpr int kB4 = kW4 + 9 * kLC;      // [1]
-constexpr int kTaps = kB4 + 1;          // [ndet]
-static_assert(kTaps == 4945, "parameter layout");
+// Flat fp32 parameter vector for C hidden channels (16 or 32): CNN weights
+// and biases, then the symmetric filter taps sym[0..ndet-1] (full filter
+// h[m] = sym[|m|]).
+//   W1[C][9] b1[C] W2[C][9][C] b2[C] W3[C][9][C] b3[C] W4[9][C] b4 taps
+struct LearnedLayout {
+    int W1, B1, W2, B2, W3, B3, W4, B4, Taps;
+};
+constexpr LearnedLayout learned_layout(int C) {
+    return LearnedLayout{0,
+                         C * 9,
+                         C * 9 + C,
+                         C * 9 + C + 9 * C * C,
+                         C * 9 + 2 * C + 9 * C * C,
+                         C * 9 + 2 * C + 18 * C * C,
+                         C * 9 + 3 * C + 18 * C * C,
+                         C * 9 + 3 * C + 18 * C * C + 9 * C,
+                         C * 9 + 3 * C + 18 * C * C + 9 * C + 1};
+}
+static_assert(learned_layout(16).Taps == 4945, "parameter layout");
+static_assert(learned_layout(32).Taps == 19105, "parameter layout");
+constexpr bool learned_channels_ok(int C) { return C == 16 || C == 32; }
 
 // Per-pixel tensors of a batch (all batch x n x n pixels).
 struct LearnedBufs {
     const float* r0;     // FBP reconstruction, fp32
-    uint16_t* a1;        // bf16 NHWC, 16 channels
+    uint16_t* a1;        // bf16 NHWC, C channels
     uint16_t* a2;
     uint16_t* a3;
     float* out;          // fp32
@@ -36,9 +44,10 @@ struct LearnedBufs {
     const uint16_t* wb2; // bf16 copies of W2, W3 ([co][t][ci])
     const uint16_t* wb3;
     const uint16_t* w1b; // bf16 W1 as [co][16] (9 taps, 7 zeros)
+    int C;               // hidden channels (16 or 32)
 };
 
-hipError_t launch_pack_weights(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s);
+hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s);
 // Training forward: a1..a3 stored (n a multiple of 16).
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s);

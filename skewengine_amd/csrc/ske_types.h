@@ -45,4 +45,15 @@ struct RowWork {
     int row;   // skewed row r < H
 };
 
+// One recorded sample point of the local Radon transform: skewed cell (i, j)
+// of sector `sec` (index into the SecDesc array); its four 64-float vectors
+// go to slot `slot` of the sample buffer.
+struct SampleReq {
+    int sec;
+    int i;
+    int j;
+    int slot;
+};
+constexpr int kMaxSamplePoints = 16;
+
 }  // namespace ske

@@ -37,6 +37,7 @@ class SkewEngine:
         if grid.ndim != 2:
             raise ValueError("grid must be 2-D (dimy, dimx)")
         self.dimy, self.dimx = grid.shape
+        self._npts = 0
         self._h = ctypes.c_void_p()
         check(lib().ske_create(self.dimx, self.dimy, grid.ctypes.data, device_id, ctypes.byref(self._h)))
 
@@ -130,6 +131,19 @@ class SkewEngine:
 
     def profile_reset(self):
         check(lib().ske_profile_reset(self._h))
+
+    def set_sample_points(self, points):
+        """Record the local Radon transform's vectors at image cells
+        [(x, y), ...] (saveSampleData, src/global.cpp:47-56)."""
+        xy = np.ascontiguousarray(np.asarray(points, np.int32).reshape(-1))
+        check(lib().ske_set_sample_points(self._h, len(xy) // 2, xy.ctypes.data if len(xy) else None))
+        self._npts = len(xy) // 2
+
+    def sample_data(self):
+        """The record: float32 (180, points, 4, 64)."""
+        out = np.zeros((180, self._npts, 4, 64), np.float32)
+        check(lib().ske_get_sample_data(self._h, out.ctypes.data))
+        return out
 
     def counter(self, name):
         """Work counter accumulated since the last profile reset (e.g.
@@ -260,6 +274,13 @@ def plan_sector(dimx, dimy, angle):
     check(lib().ske_plan_sector(dimx, dimy, angle, ctypes.byref(p), target.ctypes.data, weight.ctypes.data,
                                 first.ctypes.data, last.ctypes.data))
     return p, target, weight, first, last
+
+
+def write_sample_files(directory, data):
+    """punto<p>{o,i,l,c}.txt of showResultsRADN (src/kernelRADN.h:263-382)
+    from a record of shape (180, points, 4, 64)."""
+    data = np.ascontiguousarray(data, np.float32)
+    check(lib().ske_write_sample_files(str(directory).encode(), data.shape[1], data.ctypes.data))
 
 
 def device_count():

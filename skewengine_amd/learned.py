@@ -21,7 +21,7 @@ from . import _lib
 from ._lib import check, lib
 from .fbp import IMAGE, RECON, SINO
 
-TAPS = _lib.LEARNED_TAPS
+TAPS = _lib.LEARNED_TAPS  # taps offset of the 16-channel layout (LearnedFBP.taps_offset in general)
 R0, A1, A2, A3, DR0, FILTERED_GRAD = (_lib.LEARNED_R0, _lib.LEARNED_A1, _lib.LEARNED_A2, _lib.LEARNED_A3,
                                       _lib.LEARNED_DR0, _lib.LEARNED_FILTERED_GRAD)
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
@@ -58,13 +58,16 @@ class LearnedFBP:
     """Learned reconstruction attached to an FBP object (same geometry,
     stream and buffers: SINO in, RECON out, IMAGE the training target)."""
 
-    def __init__(self, fbp, params=None, seed=1):
+    def __init__(self, fbp, params=None, seed=1, channels=16):
         self.fbp = fbp
         self._h = ctypes.c_void_p()
-        check(lib().ske_learned_create(fbp._h, ctypes.byref(self._h)))
+        check(lib().ske_learned_create_channels(fbp._h, int(channels), ctypes.byref(self._h)))
         cnt = ctypes.c_int(0)
         check(lib().ske_learned_param_count(self._h, ctypes.byref(cnt)))
         self.param_count = cnt.value
+        c, t = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().ske_learned_layout(self._h, ctypes.byref(c), ctypes.byref(t)))
+        self.channels, self.taps_offset = c.value, t.value
         if params is not None:
             self.set_params(params)
         elif seed != 1:
@@ -126,7 +129,7 @@ class LearnedFBP:
     def download(self, which, batch=1):
         n = self.fbp.n
         if which in (A1, A2, A3):
-            out = np.empty((batch, n, n, _lib.LEARNED_CHANNELS), np.uint16)
+            out = np.empty((batch, n, n, self.channels), np.uint16)
         elif which == FILTERED_GRAD:
             out = np.empty((batch, self.fbp.n_angles, self.fbp.n_det), np.float32)
         else:
@@ -185,9 +188,11 @@ class LearnedFBP:
         return np.concatenate(p_fbp), np.concatenate(p_l)
 
 
-def default_weights(n=512, n_angles=60, n_det=725):
-    """Path of the committed trained parameters for this geometry, or None."""
-    p = os.path.join(WEIGHTS_DIR, "learned_fbp_n%d_a%d_d%d.npy" % (n, n_angles, n_det))
+def default_weights(n=512, n_angles=60, n_det=725, channels=16):
+    """Path of the committed trained parameters for this geometry and
+    channel count, or None."""
+    suffix = "" if channels == 16 else "_c%d" % channels
+    p = os.path.join(WEIGHTS_DIR, "learned_fbp_n%d_a%d_d%d%s.npy" % (n, n_angles, n_det, suffix))
     return p if os.path.exists(p) else None
 
 
@@ -202,6 +207,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--angles", type=int, default=60)
     ap.add_argument("--det", type=int, default=725)
+    ap.add_argument("--channels", type=int, default=16, choices=(16, 32))
     ap.add_argument("--steps", type=int, default=1250)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -212,7 +218,7 @@ def main(argv=None):
     ap.add_argument("--log-every", type=int, default=50)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    with FBP(a.n, a.angles, a.det, max_batch=a.batch) as f, LearnedFBP(f, seed=a.seed) as m:
+    with FBP(a.n, a.angles, a.det, max_batch=a.batch) as f, LearnedFBP(f, seed=a.seed, channels=a.channels) as m:
         if a.init:
             m.load(a.init)
         t0 = time.perf_counter()

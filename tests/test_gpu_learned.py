@@ -35,19 +35,23 @@ def _bf16_close(gpu_u16, ref, frac=1e-3):
 
 def _random_params(m, nd, n, seed):
     rng = np.random.default_rng(seed)
+    off, taps_off = L.layout(m.channels)
+    assert taps_off == m.taps_offset
     p = np.zeros(m.param_count, np.float32)
-    p[:L.TAPS] = rng.standard_normal(L.TAPS) * 0.1
+    p[:taps_off] = rng.standard_normal(taps_off) * 0.1 * np.sqrt(16.0 / m.channels)
     for name in ("b1", "b2", "b3"):
-        o, s = L.OFF[name]
+        o, s = off[name]
         p[o:o + s] = rng.uniform(-0.05, 0.2, s)
     taps = std_fbp.ramlak_taps(nd, n)[nd - 1:] * (1.0 + 0.1 * rng.standard_normal(nd))
-    p[L.TAPS:] = taps
+    p[taps_off:] = taps
     return p
 
 
-@pytest.mark.parametrize("n,nang,ndet,batch", [(64, 30, 91, 2), (96, 20, 137, 1), (160, 12, 227, 3)])
-def test_forward_layers_match_fp64(n, nang, ndet, batch):
-    with FBP(n, nang, ndet, max_batch=batch) as f, lm.LearnedFBP(f) as m:
+@pytest.mark.parametrize("n,nang,ndet,batch,channels", [(64, 30, 91, 2, 16), (96, 20, 137, 1, 16),
+                                                        (160, 12, 227, 3, 16), (64, 30, 91, 2, 32),
+                                                        (96, 20, 137, 2, 32)])
+def test_forward_layers_match_fp64(n, nang, ndet, batch, channels):
+    with FBP(n, nang, ndet, max_batch=batch) as f, lm.LearnedFBP(f, channels=channels) as m:
         prm = _random_params(m, ndet, n, seed=n)
         m.set_params(prm)
         assert np.array_equal(m.params(), prm)
@@ -61,8 +65,9 @@ def test_forward_layers_match_fp64(n, nang, ndet, batch):
         fused = f.download(RECON, batch)
     assert np.array_equal(fused, out)
     p64 = prm.astype(np.float64)
-    assert _rel(r0, L.learned_fbp(p64, sino, n)) <= 1e-4
-    g = L.unpack(p64)
+    C = channels
+    assert _rel(r0, L.learned_fbp(p64, sino, n, C=C)) <= 1e-4
+    g = L.unpack(p64, C)
     # each layer from the GPU's own input to it
     z1 = L.conv3x3(L.bf16(r0)[..., None], L.bf16(g["W1"])[:, :, None]) + g["b1"]
     _bf16_close(a[0], np.maximum(z1, 0))
@@ -74,13 +79,14 @@ def test_forward_layers_match_fp64(n, nang, ndet, batch):
     assert _rel(out, ref_out) <= 1e-5
     assert np.mean(_bf(a[1]) > 0) > 0.2  # the layers are not dead
     # whole model from the sinogram: bf16 rounding flips keep this loose
-    full = L.cnn_forward(p64, L.learned_fbp(p64, sino, n))["out"]
+    full = L.cnn_forward(p64, L.learned_fbp(p64, sino, n, C=C), C=C)["out"]
     assert _rel(out, full) <= 2e-2
 
 
-@pytest.mark.parametrize("n,nang,ndet,batch", [(64, 30, 91, 2), (128, 24, 183, 3)])
-def test_gradients_match_fp64(n, nang, ndet, batch):
-    with FBP(n, nang, ndet, max_batch=batch) as f, lm.LearnedFBP(f) as m:
+@pytest.mark.parametrize("n,nang,ndet,batch,channels", [(64, 30, 91, 2, 16), (128, 24, 183, 3, 16),
+                                                        (64, 30, 91, 2, 32)])
+def test_gradients_match_fp64(n, nang, ndet, batch, channels):
+    with FBP(n, nang, ndet, max_batch=batch) as f, lm.LearnedFBP(f, channels=channels) as m:
         prm = _random_params(m, ndet, n, seed=3 + n)
         m.set_params(prm)
         m.make_batch(batch, seed=11, i0=1e4, mu=1.0)
@@ -98,8 +104,10 @@ def test_gradients_match_fp64(n, nang, ndet, batch):
     assert abs(loss - np.sum(diff ** 2) / count) <= 1e-5 * loss
     dout = 2.0 * diff / count
     cache = {"x0": L.bf16(r0)[..., None], "a1": acts[0], "a2": acts[1], "a3": acts[2]}
-    ref, ref_dr0 = L.cnn_backward(p64, cache, dout)
-    for name, (o, s) in L.OFF.items():
+    C = channels
+    off, taps_off = L.layout(C)
+    ref, ref_dr0 = L.cnn_backward(p64, cache, dout, C=C)
+    for name, (o, s) in off.items():
         gref = np.asarray(ref[name]).reshape(-1)
         assert _rel(grad[o:o + s], gref) <= 2e-4, name
     assert _rel(dr0, ref_dr0) <= 1e-4
@@ -107,9 +115,9 @@ def test_gradients_match_fp64(n, nang, ndet, batch):
     ref_dq = np.stack([L.backproject_adjoint(dr0[b], nang, ndet) for b in range(batch)])
     assert _rel(dq, ref_dq) <= 1e-4
     dfull = sum(L.filter_grad(ref_dq[b], sino[b]) for b in range(batch))
-    assert _rel(grad[L.TAPS:], L.fold_taps(dfull)) <= 1e-4
+    assert _rel(grad[taps_off:], L.fold_taps(dfull)) <= 1e-4
     # the complete FP64 model agrees in aggregate (bf16 flips aside)
-    loss64, g64, _ = L.loss_and_grad(p64, sino, target)
+    loss64, g64, _ = L.loss_and_grad(p64, sino, target, C=C)
     assert abs(loss - loss64) <= 1e-2 * loss64
     assert np.linalg.norm(grad - g64) <= 5e-2 * np.linalg.norm(g64)
 
@@ -151,9 +159,10 @@ def test_phantom_and_noise_generators():
             assert np.allclose(cnt[sel], np.round(cnt[sel]), rtol=1e-3)
 
 
-def test_zero_output_layer_equals_plain_fbp():
+@pytest.mark.parametrize("channels", [16, 32])
+def test_zero_output_layer_equals_plain_fbp(channels):
     n, nang, ndet = 64, 30, 91
-    with FBP(n, nang, ndet, max_batch=2) as f, lm.LearnedFBP(f) as m:
+    with FBP(n, nang, ndet, max_batch=2) as f, lm.LearnedFBP(f, channels=channels) as m:
         m.init(3)  # zero output layer and Ram-Lak taps
         m.make_batch(2, seed=1)
         sino = f.download(SINO, 2)
@@ -192,10 +201,27 @@ def test_committed_weights_gain_on_held_out_phantoms():
     assert np.all(p_l > p_fbp)
 
 
+def test_committed_32_channel_weights_gain_on_held_out_phantoms():
+    path = lm.default_weights(512, 60, 725, channels=32)
+    if path is None:
+        pytest.skip("no trained 32-channel parameters committed for 512^2 / 60 angles / 725 bins")
+    with FBP(512, 60, 725, max_batch=8) as f, lm.LearnedFBP(f, channels=32) as m:
+        m.load(path)
+        p_fbp, p_l = m.evaluate(32, 8)
+    gain = float(np.mean(p_l - p_fbp))
+    print("512^2/60 angles held-out, 32 channels: plain FBP %.2f dB, learned %.2f dB, gain %.2f dB"
+          % (p_fbp.mean(), p_l.mean(), gain))
+    assert gain >= 3.0
+    assert np.all(p_l > p_fbp)
+
+
 def test_rejects_bad_geometry_and_batch():
     with FBP(48, 10, 71, max_batch=1) as f:
         with pytest.raises(Exception):
             lm.LearnedFBP(f)
+    with FBP(64, 10, 91, max_batch=1) as f:
+        with pytest.raises(Exception):
+            lm.LearnedFBP(f, channels=24)
     with FBP(64, 10, 91, max_batch=1) as f, lm.LearnedFBP(f) as m:
         with pytest.raises(Exception):
             m.reconstruct(2)

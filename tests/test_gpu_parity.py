@@ -3,7 +3,9 @@
 Bars: index maps, skewed rows, viewshed and identity outputs bit-exact
 against a one-thread oracle run (same accumulation order); Radon within
 1e-6 of the max magnitude (both compute in FP64); cepstrum per-sector values
-within 1e-4 relative.
+within 1e-4 relative except proven rounding-level ties of the peak rule
+(tests/cepstrum_ties.py), and cepstrum maps equal to the oracle's deskew and
+reduce of the GPU's per-sector values.
 """
 import os
 import tempfile
@@ -11,6 +13,7 @@ import tempfile
 import numpy as np
 import pytest
 
+import cepstrum_ties
 import oracle
 import skewengine_amd as ske
 from skewengine_amd import synth
@@ -61,9 +64,8 @@ def test_per_sector_api_matches_oracle(kernel, dem64, img64):
                 assert np.array_equal(SO, rSO), a
             elif kernel == oracle.RADON:
                 assert _rel(SO, rSO) <= 1e-6, a
-            else:
-                bad = ~np.isclose(SO, rSO, rtol=1e-4, atol=1e-3)
-                assert bad.mean() <= 1e-3, (a, bad.sum())
+            else:  # 1e-4 relative; every cell outside is a proven peak-rule tie
+                cepstrum_ties.check_sector(SO, ref)
 
 
 @pytest.mark.parametrize("kernel", [oracle.SDEM, oracle.IDENTITYDEM, oracle.RADON, oracle.CEPSTRUM])
@@ -86,7 +88,7 @@ def test_per_sector_driver_matches_oracle_execute(kernel, dem64, img64):
     elif kernel == oracle.RADON:
         assert _rel(out, ref) <= 1e-5
     else:
-        assert np.mean(out != ref) <= 1e-3
+        _assert_cepstrum_map(grid, out)
 
 
 @pytest.mark.parametrize("n", [50, 64, 70, 128])  # 50, 70: the deskew's last rows are a partial block of 16
@@ -164,12 +166,22 @@ def test_radon_c2_phantom():
     assert _rel(out, ref) <= 1e-4
 
 
+def _assert_cepstrum_map(img, out):
+    """A cepstrum map equals the oracle's deskew (max-encode) and reduce of
+    the GPU's per-sector values, each of which matches the oracle to 1e-4 or
+    is a proven peak-rule tie; where no sector holds a tie the map equals the
+    oracle's own map."""
+    secs = cepstrum_ties.gpu_sectors_checked(img)
+    assert np.array_equal(out, cepstrum_ties.deskew_reduce(img.shape[0], secs))
+    ref, _ = oracle.execute(oracle.CEPSTRUM, img, nthreads=1)
+    assert np.mean(out != ref) <= 1e-3
+
+
 def test_cepstrum_full_map():
     img = synth.blurred_texture(n=128, seed=9)[0]
     with ske.SkewEngine(img) as e:
         out = e.run(ske.KERNEL_CEPSTRUM)
-    ref, _ = oracle.execute(oracle.CEPSTRUM, img, nthreads=1)
-    assert np.mean(out != ref) <= 1e-3
+    _assert_cepstrum_map(img, out)
     # quantised to multiples of 1000 plus the angle of the last sector of the type
     assert np.all(np.mod(out, 1000) <= 179)
 
@@ -225,4 +237,4 @@ def test_run_all_sectors_radon_cepstrum_partial_row_block(kernel):
     if kernel == oracle.RADON:
         assert _rel(out, ref) <= 1e-5
     else:
-        assert np.mean(out != ref) <= 1e-3
+        _assert_cepstrum_map(img, out)

@@ -51,13 +51,14 @@ def test_exit_large_dem_matches_per_step_kernel(n, hurst):
     print("dem", n, hurst, flush=True)
     ranges = [(0, 2), (44, 47), (89, 91), (134, 136), (178, 180)]
     with ske.SkewEngine(grid) as e:
+        e.set_option(_lib.OPT_SDEM_GROUP, 1)  # one chunk per wave, 128-cell guard bands
         fast = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
-        e.set_option(_lib.OPT_SDEM_GROUP, 4)
+        e.set_option(_lib.OPT_SDEM_GROUP, 4)  # lane compaction (the default)
         packed = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
         e.set_option(_lib.OPT_SDEM_VARIANT, 2)
         full = [e.run(ske.KERNEL_SDEM, s0, s1) for s0, s1 in ranges]
     for (s0, s1), a, b, c in zip(ranges, fast, packed, full):
-        assert np.array_equal(a, c), (n, s0, s1)
+        assert np.array_equal(a, c), (n, s0, s1, "group 1")
         assert np.array_equal(b, c), (n, s0, s1, "group 4")
 
 

@@ -93,3 +93,34 @@ def test_plan_sector_matches_oracle_maps(n):
         ske.plan_sector(n, n, 180)
     with pytest.raises(ske.SkeError):
         ske.plan_sector(n, n + 1, 0)
+
+
+def test_sample_file_layout(tmp_path):
+    """showResultsRADN's text layout (src/kernelRADN.h:280-360): for each
+    point and vector two blocks of 180 lines, values [32 + j] then [32 - j]
+    for j = 0..31; the spectrum file ('l') repeats the [32 - j] block."""
+    npts = 2
+    data = np.arange(180 * npts * 4 * 64, dtype=np.float32).reshape(180, npts, 4, 64) * 0.5
+    ske.write_sample_files(tmp_path, data)
+    names = sorted(os.listdir(tmp_path))
+    assert names == sorted("punto%d%s.txt" % (p, k) for p in (1, 2) for k in "oilc")
+    for p in range(npts):
+        for v, kind in enumerate("oilc"):
+            text = open(tmp_path / ("punto%d%s.txt" % (p + 1, kind))).read()
+            lines = text.split("\n")
+            assert len(lines) == 361 and lines[-1] == "" and lines[0].endswith(" ")
+            got = np.array([[float(t) for t in ln.split()] for ln in lines[:-1]])
+            j = np.arange(32)
+            up, down = data[:, p, v][:, 32 + j], data[:, p, v][:, 32 - j]
+            exp = np.concatenate([down if kind == "l" else up, down])
+            assert np.array_equal(got, exp.astype(np.float64)), (p, kind)
+    # values print like std::ostream's default float format (%g, 6 digits)
+    one = np.zeros((180, 1, 4, 64), np.float32)
+    one[0, 0, 0, 32] = 1234567.0
+    one[0, 0, 0, 33] = 1e-5
+    one[0, 0, 0, 34] = -0.0
+    ske.write_sample_files(tmp_path, one)
+    first = open(tmp_path / "punto1o.txt").readline()
+    assert first.startswith("1.23457e+06 1e-05 -0 0 ")
+    with pytest.raises(ske.SkeError):
+        ske.write_sample_files(tmp_path / "missing_dir", one)

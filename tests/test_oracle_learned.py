@@ -108,3 +108,38 @@ def test_zero_output_layer_gives_plain_fbp(bf):
     r0 = L.learned_fbp(prm, sino, n)
     out = L.cnn_forward(prm, r0, bf)["out"]
     assert np.allclose(out[0], std_fbp.fbp(sino[0], n), rtol=0, atol=1e-9 * np.abs(out).max())
+
+
+def test_gradient_matches_central_differences_32_channels():
+    """The same pin for the 32-channel layout (W1[32][9] ... W4[9][32] b4 taps)."""
+    off, taps = L.layout(32)
+    assert taps == 19105 and off["W2"] == (320, 9216)
+    n, na, nd = 32, 6, 45
+    rng = np.random.default_rng(5)
+    prm = np.zeros(taps + nd)
+    prm[:taps] = rng.standard_normal(taps) * 0.1
+    prm[taps:] = std_fbp.ramlak_taps(nd, n)[nd - 1:]
+    sino = rng.standard_normal((1, na, nd))
+    tgt = rng.standard_normal((1, n, n))
+    _, g, _ = L.loss_and_grad(prm, sino, tgt, bf=False, C=32)
+    assert g.size == taps + nd
+    idx = []
+    for name, (o, s) in off.items():
+        idx += list(o + rng.choice(s, size=min(s, 2), replace=False))
+    idx += [taps, taps + 7]
+    e = 1e-6
+    for i in idx:
+        p1, p2 = prm.copy(), prm.copy()
+        p1[i] += e
+        p2[i] -= e
+        fd = (L.loss_and_grad(p1, sino, tgt, bf=False, C=32)[0] - L.loss_and_grad(p2, sino, tgt, bf=False, C=32)[0]) \
+            / (2 * e)
+        assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_layout_sizes():
+    for C, taps in ((16, 4945), (32, 19105)):
+        off, t = L.layout(C)
+        assert t == taps
+        assert off["W1"] == (0, 9 * C) and off["b1"] == (9 * C, C)
+        assert off["W2"] == (10 * C, 9 * C * C) and off["W4"][1] == 9 * C and off["b4"] == (t - 1, 1)
