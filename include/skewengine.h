@@ -73,6 +73,23 @@ int run(int argc, char **argv);
 int ske_execute(int kernel_id, int dimx, int dimy, const float *input, float *output, float povh,
                 int sector_begin, int sector_end);
 
+/* ske_execute without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture elsewhere in their program, tight serving loops, harnesses
+ * that forbid hipMalloc).  One n x n grid (2 <= n <= SKE_NOALLOC_MAX_SIDE),
+ * d_input and d_output device buffers of n*n floats on the current device,
+ * sectors [sector_begin, sector_end), radon_local = the LRT flag of
+ * SKE_OPT_RADON_LOCAL; stream is a hipStream_t (NULL: the null stream).
+ * Every other buffer lives in one static device array of
+ * SKE_NOALLOC_SCRATCH_MB MiB per GPU, held until the process ends.  Calls
+ * no hipMalloc/hipFree; host-side planning uses ordinary host memory.
+ * Returns when the result is in d_output.  Calls on one GPU share its array:
+ * calls that overlap on one GPU race.  Results equal ske_execute's. */
+#define SKE_NOALLOC_SCRATCH_MB 96
+#define SKE_NOALLOC_MAX_SIDE 1024
+int ske_execute_noalloc(int kernel_id, int n, const float *d_input, float *d_output, float povh,
+                        int sector_begin, int sector_end, int radon_local, void *stream);
+
 /* skewEngine<float>(dimx, dimy, inData, isGPU=true, deviceId)
  * (src/skewEngine.cuh:281,461) together with allocate/prepare (:388-411,
  * :962-987) and skewAlloc (:418-456).  `input` is a host grid; the engine

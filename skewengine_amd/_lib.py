@@ -24,7 +24,7 @@ MAX_SECTOR = 180
 
 # Every symbol include/skewengine.h declares.
 EXPORTS = [
-    "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_create",
+    "ske_last_error", "ske_version", "ske_device_count", "run", "ske_execute", "ske_execute_noalloc", "ske_create",
     "ske_create_device", "ske_destroy", "ske_set_kernel", "ske_set_device_operator", "ske_skew",
     "ske_kernel", "ske_deskew", "ske_reduce", "ske_get_params", "ske_get_tables", "ske_plan_sector", "ske_set_sample_points",
     "ske_get_sample_data", "ske_write_sample_files",
@@ -84,6 +84,8 @@ def lib():
     L.run.argtypes = [ctypes.c_int, P(ctypes.c_char_p)]
     L.ske_execute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_int,
                               ctypes.c_int]
+    L.ske_execute_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, vp]
     L.ske_create.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, P(vp)]
     L.ske_create_device.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, P(vp)]
     L.ske_destroy.argtypes = [vp]

@@ -23,7 +23,7 @@ ARCH = os.environ.get("SKE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels_core.hip", "kernels_sdem.hip", "kernels_radn.hip", "kernels_blur.hip", "kernels_fbp.hip",
                "kernels_learned.hip"]
-CPP_SOURCES = ["planner.cpp", "engine.cpp", "fbp.cpp", "learned.cpp", "png_io.cpp", "driver.cpp"]
+CPP_SOURCES = ["planner.cpp", "engine.cpp", "noalloc.cpp", "fbp.cpp", "learned.cpp", "png_io.cpp", "driver.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-I" + INCLUDE, "-I" + CSRC]
