@@ -25,7 +25,7 @@
 namespace ske {
 
 // The projector reads a copy of each image with a zero border of kProjPad
-// pixels, so the bilinear taps need no bounds tests: the samples of
+// pixels, so the bilinear taps (ray_sample) need no bounds tests: the samples of
 // ray_range lie within 1 + 2 delta of the centre along each axis, i.e. their
 // taps within 3 pixels of the image (checked exhaustively on the CPU for
 // n = 2..1024, 2..2n+3 bins and arbitrary angles), and zeros there equal the
@@ -38,18 +38,6 @@ __global__ __launch_bounds__(256) void k_fbp_pad(const float* __restrict__ img, 
     const size_t b = e / per;
     const int r = (int)((e - b * per) / np) - kProjPad, c = (int)(e % np) - kProjPad;
     pad[e] = (r >= 0 && r < n && c >= 0 && c < n) ? img[b * n * n + (size_t)r * n + c] : 0.0f;
-}
-
-__device__ __forceinline__ float bilinear_padded(const float* __restrict__ pim, int np, float u, float v) {
-    // u: column coordinate, v: row coordinate, pixel centres at integers
-    const float fu = floorf(u), fv = floorf(v);
-    // 24-bit multiply: row and np are below 2^14, the offset below 2^31
-    const float* r = pim + (__mul24((int)fv + kProjPad, np) + ((int)fu + kProjPad));
-    const float wu = u - fu, wv = v - fv;
-    const float a = r[0], b = r[1], c = r[np], d = r[np + 1];
-    const float top = a + wu * (b - a);
-    const float bot = c + wu * (d - c);
-    return top + wv * (bot - top);
 }
 
 // Sample range [i0, i1] of the ray of bin k at angle (c, sn): the samples
@@ -83,14 +71,30 @@ __device__ __forceinline__ void ray_range(const FbpGeom& g, float s, float c, fl
     }
 }
 
-__device__ __forceinline__ float ray_sample(const float* __restrict__ pim, int np, const FbpGeom& g, float s,
-                                            float c, float sn, int i) {
+// Sample i of the ray of bin offset s at angle (c, sn) in pixel coordinates:
+// with t = (i - half) delta, u = (s c - t sn + 1)/delta - 1/2 = U0 - i sn and
+// v = (1 - s sn - t c)/delta - 1/2 = V0 - i c, so a sample costs two fma.
+struct RayCoord {
+    float u0, v0;
+};
+__device__ __forceinline__ RayCoord ray_coord(const FbpGeom& g, float s, float c, float sn) {
     const float half = 0.5f * (float)(g.nt - 1);
     const float inv = 1.0f / g.delta;
-    const float t = ((float)i - half) * g.delta;
-    const float x = s * c - t * sn;
-    const float y = s * sn + t * c;
-    return bilinear_padded(pim, np, (x + 1.0f) * inv - 0.5f, (1.0f - y) * inv - 0.5f);
+    return RayCoord{((s * c + 1.0f) * inv - 0.5f) + half * sn, ((1.0f - s * sn) * inv - 0.5f) + half * c};
+}
+
+__device__ __forceinline__ float ray_sample(const float* __restrict__ pim, int np, const RayCoord& rc, float c,
+                                            float sn, int i) {
+    const float fi = (float)i;
+    const float u = __builtin_fmaf(-fi, sn, rc.u0), v = __builtin_fmaf(-fi, c, rc.v0);
+    const float fu = floorf(u), fv = floorf(v);
+    // 24-bit multiply: row and np are below 2^14, the offset below 2^31
+    const float* r = pim + (__mul24((int)fv + kProjPad, np) + ((int)fu + kProjPad));
+    const float wu = u - fu, wv = v - fv;
+    const float a = r[0], b = r[1], cc = r[np], d = r[np + 1];
+    const float top = __builtin_fmaf(wu, b - a, a);
+    const float bot = __builtin_fmaf(wu, d - cc, cc);
+    return __builtin_fmaf(wv, bot - top, top);
 }
 
 // One wave per 8 bins of one angle and slice: lane (kb, ib) = (lane % 8,
@@ -118,6 +122,7 @@ __global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ p
     int i0, i1;
     ray_range(g, s, t2.x, t2.y, i0, i1);
     if (kbase + kb >= g.ndet) i1 = i0 - 1;  // padding lanes: no samples
+    const RayCoord rc = ray_coord(g, s, t2.x, t2.y);
     // common range of the wave's 8 bins (every lane holds its own bin's range)
     int lo = i1 >= i0 ? i0 : INT_MAX, hi = i1 >= i0 ? i1 : INT_MIN;
 #pragma unroll
@@ -136,7 +141,7 @@ __global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ p
             for (int u = 0; u < 4; ++u) {
                 const int iu = i + 8 * u;
                 v[u] = 0.0f;
-                if (iu >= i0 && iu <= i1) v[u] = ray_sample(im, np, g, s, t2.x, t2.y, iu);  // lanes masked: no load
+                if (iu >= i0 && iu <= i1) v[u] = ray_sample(im, np, rc, t2.x, t2.y, iu);  // lanes masked: no load
             }
             acc += v[0];
             acc += v[1];
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ p
             acc += v[3];
         }
         for (; i <= hi; i += 8)
-            if (i >= i0 && i <= i1) acc += ray_sample(im, np, g, s, t2.x, t2.y, i);
+            if (i >= i0 && i <= i1) acc += ray_sample(im, np, rc, t2.x, t2.y, i);
     }
     acc += __shfl_xor(acc, 8, 64);
     acc += __shfl_xor(acc, 16, 64);
@@ -224,7 +229,8 @@ constexpr int kBpChunk = 128;
 __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict__ filt, float* __restrict__ img,
                                                          const float2* __restrict__ cs, FbpGeom g, float scale) {
     __shared__ float q[kBpChunk * kBpWin];
-    __shared__ int base[kBpChunk];
+    __shared__ float2 csi[kBpChunk];  // (cos, sin) / delta of each angle of the chunk
+    __shared__ int base[kBpChunk];    // window offset: aa * kBpWin - first bin of the window
     const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
     const int j = j0 + (threadIdx.x & 15);
     const int i = i0 + (threadIdx.x >> 4);
@@ -242,27 +248,29 @@ __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict
         __syncthreads();
         for (int t = threadIdx.x; t < na; t += blockDim.x) {
             const float2 t2 = cs[a0 + t];
-            base[t] = (int)floorf((xc * t2.x + yc * t2.y) * inv + g.det_c) - kBpWin / 2;
+            const float2 ci = make_float2(t2.x * inv, t2.y * inv);
+            csi[t] = ci;
+            // bin position of the tile centre, as the pixels compute theirs
+            const float kc = __builtin_fmaf(xc, ci.x, __builtin_fmaf(yc, ci.y, g.det_c));
+            base[t] = t * kBpWin - ((int)floorf(kc) - kBpWin / 2);
         }
         __syncthreads();
         for (int t = threadIdx.x; t < na * kBpWin; t += blockDim.x) {
             const int aa = t / kBpWin;
-            const int k = base[aa] + (t % kBpWin);
+            const int k = (t % kBpWin) - (base[aa] - aa * kBpWin);
             q[t] = (k >= 0 && k < g.ndet) ? fb[(size_t)(a0 + aa) * g.ndet + k] : 0.0f;
         }
         __syncthreads();
         if (in) {
             for (int aa = 0; aa < na; ++aa) {
-                const float2 t2 = cs[a0 + aa];
-                const float sp = x * t2.x + y * t2.y;
-                const float kf = sp * inv + g.det_c;
+                const float2 ci = csi[aa];
+                const float kf = __builtin_fmaf(x, ci.x, __builtin_fmaf(y, ci.y, g.det_c));
                 const float fk = floorf(kf);
-                const int k0 = (int)fk;
                 const float w = kf - fk;
-                const int r = aa * kBpWin + k0 - base[aa];  // in [4, 28) (tile radius < 12 bins)
+                const int r = (int)fk + base[aa];  // aa * kBpWin + [4, 28): the tile radius is < 12 bins
                 const float v0 = q[r];
                 const float v1 = q[r + 1];
-                acc += v0 + w * (v1 - v0);
+                acc += __builtin_fmaf(w, v1 - v0, v0);
             }
         }
     }

@@ -1,11 +1,11 @@
 """CPU check of the invariant the standard-geometry projector relies on
-(kernels_fbp.hip: k_fbp_pad, bilinear_padded): for every ray sample that
+(kernels_fbp.hip: k_fbp_pad, ray_sample): for every ray sample that
 ray_range admits, the four bilinear taps lie within 3 pixels of the image, so
 a zero border of kProjPad = 4 pixels replaces all bounds tests.
 
 The sample coordinates are recomputed here in float32 with the kernel's
-operation order (no FMA contraction, IEEE division), vectorised over bins and
-samples, for small and odd image sides, detector counts from 2 to 2n+3 and
+operation order (two explicit fma per sample, otherwise no contraction, IEEE
+division), vectorised over bins and samples, for small and odd image sides, detector counts from 2 to 2n+3 and
 angles on the axes, the diagonals and at random."""
 import math
 
@@ -53,11 +53,12 @@ def _footprint(n, ndet, c, sn):
     inv = f32(1.0) / delta
     for b in np.nonzero(ok & (i1 >= i0))[0]:
         i = np.arange(i0[b], i1[b] + 1).astype(np.float32)
-        t = (i - half) * delta
-        x = s[b] * c - t * sn
-        y = s[b] * sn + t * c
-        u = (x + f32(1.0)) * inv - f32(0.5)
-        v = (f32(1.0) - y) * inv - f32(0.5)
+        # ray_coord / ray_sample: u = fma(-i, sn, U0), v = fma(-i, c, V0); the
+        # product of two floats is exact in double, then one rounding to float
+        u0 = ((s[b] * c + f32(1.0)) * inv - f32(0.5)) + half * sn
+        v0 = ((f32(1.0) - s[b] * sn) * inv - f32(0.5)) + half * c
+        u = (-i.astype(np.float64) * np.float64(sn) + np.float64(u0)).astype(np.float32)
+        v = (-i.astype(np.float64) * np.float64(c) + np.float64(v0)).astype(np.float32)
         ju, iv = np.floor(u).astype(np.int64), np.floor(v).astype(np.int64)
         lo = min(lo, int(ju.min()), int(iv.min()))
         hi = max(hi, int(ju.max()) + 1, int(iv.max()) + 1)
