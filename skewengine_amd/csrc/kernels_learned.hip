@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fbp.h"
 #include "learned.h"
@@ -33,11 +34,10 @@ namespace ske {
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 // fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
-    bf16x2 v;
-    v.x = (__bf16)lo;
-    v.y = (__bf16)hi;
-    return __builtin_bit_cast(uint32_t, v);
+    const f32x2 f = {lo, hi};  // one v_cvt_pk_bf16_f32 for the pair
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }  // value as bf16
@@ -81,19 +81,35 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // runs two channel blocks on the same pixel operand.  Lane layout: A[i][k]
 // at lane i + 16*(k/8), element k%8; B[k][j] at lane j + 16*(k/8), element
 // k%8; D[i][j] at lane j + 16*(i/4), register i%4.
-//   conv 1 -> C: k = 4*ky + kx over kx = 0..3 (the kx = 3 weight is zero)
-//   for the three rows, then zeros: one MFMA per group and channel block;
-//   the operand is read as bf16 pairs (x, x+1) and (x+2, x+3) from two
-//   copies of the r0 tile packed at even and at odd columns;
+//   conv 1 -> C: v_mfma_f32_16x16x16_bf16 (4 k per lane) with k = 4*ky + kx
+//   over kx = 0..3 (the kx = 3 weight is zero) for the three rows, then
+//   zeros: one MFMA per group and channel block; the operand is read as
+//   bf16 pairs (x, x+1) and (x+2, x+3) from two copies of the r0 tile
+//   packed at even and at odd columns;
 //   conv C -> C: K blocks of 32 = (32/C) taps x C channels, in tap order:
 //   the tap pairs (0,1), (2,3), (4,5), (6,7), (8,-) for C = 16, the nine
 //   taps for C = 32.
-//   conv C -> 1: v_dot2c_f32_bf16 on channel pairs, channels [0, C/2) and
-//   [C/2, C) in two partial sums (taps in order, pairs in order), then
-//   (s0+s1) + b4.
-// Activation tiles are bf16 NHWC in LDS at a pitch of C values per pixel
-// (no pad: a 48-byte pitch at C = 16 avoids bank conflicts but costs two
-// workgroups per CU, and measured slower).
+//   conv C -> 1: the K blocks of conv C -> C with W4 in every row of A
+//   (every row of D is the output), then + b4.
+// Activation tiles are bf16 in LDS, one plane per 16 channels: each region
+// pixel (row pitch: the region side) holds 32 bytes
+// per plane, two 16-byte halves of 8 channels, swapped in columns with bit
+// 2 set (chunk_off).  LDS on gfx950, as measured with SQ_LDS_BANK_CONFLICT
+// on probe patterns: a ds_read_b128 takes four cycles of 16 lanes on 64
+// banks, the lanes of a cycle being the 4-lane blocks {0,3,5,6},
+// {1,2,4,7}, {8,11,13,14}, {9,10,12,15}; ds_write_b64 four cycles of 16
+// consecutive lanes on 32 banks; ds_write_b128 eight cycles of 8 (idle
+// lanes' cycles included); ds_read_b32 two cycles of 32 lanes on 32 banks.
+// With this layout:
+//  - a layer's groups are the first 16 pixels of a row (then blocks of the
+//    leftover columns), whose operand reads meet no bank twice;
+//  - every lane stores 8 bytes of a chunk after one v_permlane16_swap
+//    (store_block); a store cycle's 16 lanes meet each bank at most twice
+//    (the minimum for 16 pixels' same 8 bytes at a 32-byte pitch);
+//  - a lane's addresses depend on its column and tap only, so the row loop
+//    of a layer adds immediate offsets and no address arithmetic.
+// (A plain NHWC layout with 4-channel stores and groups that wrap rows
+// spends 52% of its LDS cycles in bank conflicts, measured.)
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kFT = 16;          // output tile side
@@ -103,6 +119,13 @@ constexpr int kF2 = kFT + 4;     // a2
 constexpr int kF3 = kFT + 2;     // a3
 constexpr int kFWaves = 4;
 constexpr int kS0 = 12;          // bf16 pairs per row of the packed r0 tile
+// The odd-column copy of the r0 tile starts 16 words (mod 32) after the
+// even-column one, so the conv 1 -> C operand reads (4 bytes a lane, 32
+// lanes a cycle on 32 banks) of neighbouring columns fall on different banks.
+constexpr int kS0Odd = kF0 * kS0 + 16;
+constexpr int kP3 = kF3;  // row pitch of the a3 region
+constexpr int kPlA = kF1 * kF1 * 16;  // plane stride (elements) of the a1/a3 region
+constexpr int kPlB = kF2 * kF2 * 16;  // of the a2 region
 
 template <int C>
 struct MidK {
@@ -114,26 +137,17 @@ struct MidK {
 };
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
-__device__ __forceinline__ float dot2_bf16(uint32_t w, uint32_t x, float acc) {
-    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), __builtin_bit_cast(bf16x2, x), acc, false);
-}
 
-// B operand of conv 1 -> C for the a1-region pixel (y, x): k = 8*kb ..
-// 8*kb+7 with k = 4*ky + kx, i.e. the pairs (x, x+1), (x+2, x+3) of tile
-// rows y, y+1 (kb = 0) or y+2 (kb = 1); `se`/`so` hold the pairs starting at
-// even / odd columns.
-__device__ __forceinline__ bf16x8 in_operand(const uint32_t* se, const uint32_t* so, int y, int x, int kb) {
-    // branch-free: every lane reads rows y + 2*(kb == 1) and y + 1 (both
-    // inside the 24-row tile for y < 22) and zeroes what its k block lacks
-    const uint32_t* t = ((x & 1) ? so : se) + (x >> 1);
-    const int ra = (y + (kb == 1 ? 2 : 0)) * kS0, rb = (y + 1) * kS0;
-    const uint32_t a0 = t[ra], a1 = t[ra + 1], b0 = t[rb], b1 = t[rb + 1];
-    uint4 v;
-    v.x = kb < 2 ? a0 : 0u;
-    v.y = kb < 2 ? a1 : 0u;
-    v.z = kb == 0 ? b0 : 0u;
-    v.w = kb == 0 ? b1 : 0u;
-    return as_bf16x8(v);
+// B operand of conv 1 -> C (v_mfma_f32_16x16x16_bf16, k = 4*ky + kx) for an
+// a1-region pixel (y, x): lane block kb < 3 holds the bf16 pairs (x, x+1),
+// (x+2, x+3) of tile row y + kb, block 3 zeros.  `p` points at the lane's
+// pair in row y + min(kb, 2) of the copy packed at the parity of x (the
+// even-column copy at s0, the odd one at s0 + kS0Odd); `off` adds whole
+// rows; `m` is all ones for kb < 3, else zero.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ s16x4 in_operand(const uint32_t* p, int off, uint32_t m) {
+    const uint2 v = make_uint2(p[off] & m, p[off + 1] & m);
+    return __builtin_bit_cast(s16x4, v);
 }
 
 // relu, bf16, and a lane's 4 channels of one pixel (8 bytes); `keep` false
@@ -145,13 +159,34 @@ __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
     const s16x2 m = __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0});
     return __builtin_bit_cast(uint32_t, m);
 }
-__device__ __forceinline__ void store4(uint16_t* dst, const f32x4& acc, bool keep) {
-    uint2 v;
-    v.x = relu_bf16x2(pack_bf16(acc[0], acc[1]));
-    v.y = relu_bf16x2(pack_bf16(acc[2], acc[3]));
-    if (!keep) v = make_uint2(0, 0);
-    *(uint2*)dst = v;
+// Offset (elements) of 8-channel chunk c (0 or 1) of the pixel at row y,
+// column x of a planar region with row pitch `pitch`, within its plane: the
+// two 16-byte halves are swapped in columns with bit 2 set.  Eight
+// consecutive pixels' same chunk then covers 128 distinct bytes (a store
+// cycle), and the four 4-pixel blocks of a read cycle (two of chunk 0, two
+// of chunk 1, 8 columns apart) fall on distinct banks.  The swap depends on
+// the column only, so a lane that walks down a column keeps its offsets.
+__device__ __forceinline__ int chunk_off(int y, int x, int pitch, int c) {
+    return (y * pitch + x) * 16 + 8 * ((c ^ (x >> 2)) & 1);
 }
+
+// One MFMA result block (16 channels 16h .. 16h+15 of 16 pixels; lane
+// (j, kb) holds channels 16h + 4kb .. +3 of pixel j) to a planar LDS region:
+// the lane's two packed bf16 pairs x, y are exchanged between rows 2r and
+// 2r + 1 (v_permlane16_swap: the even row gets the odd row's x, the odd row
+// the even row's y), and every lane stores 8 bytes at `dst` = byte 8 (kb & 1)
+// of chunk kb / 2.  Within a chunk the channels are thus stored in the order
+// 0, 1, 4, 5, 2, 3, 6, 7 (pair_swap restores the order); the weights of the
+// next layer are read in the same order.
+__device__ __forceinline__ void store_block(uint16_t* dst, const f32x4& acc, bool keep, bool store) {
+    uint32_t x = relu_bf16x2(pack_bf16(acc[0], acc[1]));
+    uint32_t y = relu_bf16x2(pack_bf16(acc[2], acc[3]));
+    if (!keep) x = y = 0u;
+    const auto sw = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+    if (store) *(uint2*)dst = make_uint2(sw[0], sw[1]);
+}
+// The stored channel order of a chunk and back: swap its middle 4-byte words.
+__device__ __forceinline__ uint4 pair_swap(uint4 v) { return make_uint4(v.x, v.z, v.y, v.w); }
 
 // A operands of a conv C -> C layer: lane (co = l%16, kb = l/16) of channel
 // block h holds W[16h + co][tap][ch .. ch+7] for K block tp, with
@@ -166,84 +201,201 @@ __device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[MidK<C>::kHalves][M
         for (int tp = 0; tp < K::kBlocks; tp++) {
             const int tap = K::kTapsPer * tp + kb / K::kChunks;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (tap < 9) v = *(const uint4*)(wb + ((16 * h + co) * 9 + tap) * C + 8 * (kb % K::kChunks));
+            if (tap < 9) v = pair_swap(*(const uint4*)(wb + ((16 * h + co) * 9 + tap) * C + 8 * (kb % K::kChunks)));
             w[h][tp] = as_bf16x8(v);
         }
 }
 
-// One conv C -> C layer from the LDS region `in` (side wi) to `out` (side
-// wi - 2) whose top-left pixel is image pixel (oy, ox).  A wave runs its
-// groups g and g + kFWaves together (two independent accumulators per
-// channel block).  Lanes of an absent tap 9 read tap 8 again; their weights
-// are zero.
-template <int C, bool kEdge>
-__device__ __forceinline__ void mid_layer(const uint16_t* in, uint16_t* out, int wi, int oy, int ox, int n,
-                                          const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
-                                          const f32x4 (&bias)[MidK<C>::kHalves], int wave, int j16, int kb) {
+// The leftover columns 16 .. WO-1 of a mid layer with output side WO (20 or
+// 18), in groups of 16 pixels: 16 / (WO - 16) rows a group, taken in an
+// order that keeps each read cycle's lanes on distinct banks at input
+// pitches 22 and 20.
+template <int WO>
+struct MidLeft {
+    static constexpr int kK = WO - 16;                 // leftover columns
+    static constexpr int kRows = 16 / kK;              // rows per group
+    static constexpr int kGroups = (WO + kRows - 1) / kRows;
+};
+// NR rows of a mid layer's first 16 columns at once (NR independent
+// accumulator chains): rows 4 (K0 + r), r < NR, after the lane's own, of
+// the input at `rin` (per tap) and the output at `rout`; y0 is the image row
+// of the lane's first output row.
+template <int C, bool kEdge, int WI, int PO, int NR, int K0>
+__device__ __forceinline__ void mid_rows(const uint16_t* const (&rin)[MidK<C>::kBlocks], uint16_t* rout, int pout,
+                                         const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
+                                         const f32x4 (&bias)[MidK<C>::kHalves], int y0, int n, bool xin) {
     using K = MidK<C>;
-    const int wo = wi - 2, count = wo * wo;
-    int toff[K::kBlocks];
+    f32x4 acc[NR][K::kHalves];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int h = 0; h < K::kHalves; h++) acc[r][h] = bias[h];
 #pragma unroll
     for (int tp = 0; tp < K::kBlocks; tp++) {
-        const int tap = min(K::kTapsPer * tp + kb / K::kChunks, 8);
-        toff[tp] = ((tap / 3) * wi + tap % 3) * C + 8 * (kb % K::kChunks);
+        bf16x8 v[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) v[r] = as_bf16x8(*(const uint4*)(rin[tp] + 4 * (K0 + r) * WI * 16));
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int h = 0; h < K::kHalves; h++)
+                acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v[r], acc[r][h], 0, 0, 0);
     }
-    for (int g0 = wave; g0 * 16 < count; g0 += 2 * kFWaves) {
-        const int g1 = g0 + kFWaves;
-        const int p0 = g0 * 16 + j16, p1 = g1 * 16 + j16;
-        const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
-        const int y0 = q0 / wo, x0 = q0 % wo, y1 = q1 / wo, x1 = q1 % wo;
-        const uint16_t* b0 = in + (y0 * wi + x0) * C;
-        f32x4 acc0[K::kHalves];
 #pragma unroll
-        for (int h = 0; h < K::kHalves; h++) acc0[h] = bias[h];
-        if (g1 * 16 < count) {  // two groups (wave-uniform branch)
-            const uint16_t* b1 = in + (y1 * wi + x1) * C;
-            f32x4 acc1[K::kHalves];
+    for (int r = 0; r < NR; r++) {
+        const int y = y0 + 4 * (K0 + r);
+        const bool keep = xin && (!kEdge || (y >= 0 && y < n));
 #pragma unroll
-            for (int h = 0; h < K::kHalves; h++) acc1[h] = bias[h];
-#pragma unroll
-            for (int tp = 0; tp < K::kBlocks; tp++) {
-                const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + toff[tp]));
-                const bf16x8 v1 = as_bf16x8(*(const uint4*)(b1 + toff[tp]));
-#pragma unroll
-                for (int h = 0; h < K::kHalves; h++) {
-                    acc0[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v0, acc0[h], 0, 0, 0);
-                    acc1[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v1, acc1[h], 0, 0, 0);
-                }
-            }
-            const int iy = oy + y1, ix = ox + x1;
-            const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
-            if (p1 < count)
-#pragma unroll
-                for (int h = 0; h < K::kHalves; h++) store4(out + p1 * C + 16 * h + 4 * kb, acc1[h], keep);
-        } else {
-#pragma unroll
-            for (int tp = 0; tp < K::kBlocks; tp++) {
-                const bf16x8 v0 = as_bf16x8(*(const uint4*)(b0 + toff[tp]));
-#pragma unroll
-                for (int h = 0; h < K::kHalves; h++)
-                    acc0[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v0, acc0[h], 0, 0, 0);
-            }
-        }
-        const int iy = oy + y0, ix = ox + x0;
-        const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
-        if (p0 < count)
-#pragma unroll
-            for (int h = 0; h < K::kHalves; h++) store4(out + p0 * C + 16 * h + 4 * kb, acc0[h], keep);
+        for (int h = 0; h < K::kHalves; h++) store_block(rout + h * pout + 4 * (K0 + r) * PO * 16, acc[r][h], keep, true);
     }
 }
 
-// Copy the centre 16 x 16 pixels of an LDS region (side wi, centre offset
-// `off`) to the NHWC image `dst` (training: the backward kernels read it).
+// One conv C -> C layer from the LDS region `in` (side and row pitch WI,
+// plane stride pin) to `out` (side WI - 2, row pitch PO, plane stride pout)
+// whose top-left pixel is image pixel (oy, ox).  Groups of 16 pixels: the
+// first 16 columns of each row -- wave w takes rows w, w + 4, w + 8, ...,
+// two at a time (two independent accumulators per channel block), with its
+// operand and store addresses fixed per lane and the rows as immediate
+// offsets -- then the leftover columns.  Lanes of an absent tap 9 read tap 8
+// again; their weights are zero.
+template <int C, bool kEdge, int WI, int PO>
+__device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t* out, int pout, int oy, int ox,
+                                          int n, const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
+                                          const f32x4 (&bias)[MidK<C>::kHalves], int wave, int j16, int kb) {
+    using K = MidK<C>;
+    constexpr int WO = WI - 2;
+    const int ch = kb % K::kChunks;  // this lane's 8-channel chunk of every tap
+    int tpy[K::kBlocks], tpx[K::kBlocks];
+#pragma unroll
+    for (int tp = 0; tp < K::kBlocks; tp++) {
+        const int tap = min(K::kTapsPer * tp + kb / K::kChunks, 8);
+        tpy[tp] = tap / 3;
+        tpx[tp] = tap % 3;
+    }
+    const uint16_t* const cin = in + (ch >> 1) * pin;
+    {
+        const uint16_t* rin[K::kBlocks];  // row `wave` of this lane's taps
+#pragma unroll
+        for (int tp = 0; tp < K::kBlocks; tp++) rin[tp] = cin + chunk_off(wave + tpy[tp], j16 + tpx[tp], WI, ch);
+        uint16_t* const rout = out + chunk_off(wave, j16, PO, kb >> 1) + 4 * (kb & 1);
+        const bool xin = !kEdge || (ox + j16 >= 0 && ox + j16 < n);
+        // rows wave, wave + 4, wave + 8, wave + 12 (inside any layer of side >= 16), then wave + 16
+        mid_rows<C, kEdge, WI, PO, 4, 0>(rin, rout, pout, w, bias, oy + wave, n, xin);
+        if (wave + 16 < WO) mid_rows<C, kEdge, WI, PO, 1, 4>(rin, rout, pout, w, bias, oy + wave, n, xin);
+    }
+    {   // leftover columns: lane j's pixel row is kRows * g + ro(j); only the
+        // 18-wide layer's last group has rows past the side (read clamped, not stored)
+        using M = MidLeft<WO>;
+        const int r = j16 / M::kK;
+        const int ro = M::kK == 4 ? (r ^ (r >> 1)) : ((r & 3) * 2 + (r >> 2));
+        const int y0 = M::kRows * wave + ro, x = 16 + j16 % M::kK;
+        const int yc = min(y0, WO - 1);
+        const uint16_t* rin[K::kBlocks];
+#pragma unroll
+        for (int tp = 0; tp < K::kBlocks; tp++) rin[tp] = cin + chunk_off(yc + tpy[tp], x + tpx[tp], WI, ch);
+        uint16_t* const o = out + chunk_off(yc, x, PO, kb >> 1) + 4 * (kb & 1);
+#pragma unroll
+        for (int mm = 0; mm < (M::kGroups + kFWaves - 1) / kFWaves; mm++) {
+            if (wave + kFWaves * mm >= M::kGroups) break;  // wave-uniform
+            const int y = y0 + kFWaves * M::kRows * mm;
+            constexpr int kStep = kFWaves * M::kRows;      // rows between a lane's groups
+            f32x4 acc[K::kHalves];
+#pragma unroll
+            for (int h = 0; h < K::kHalves; h++) acc[h] = bias[h];
+#pragma unroll
+            for (int tp = 0; tp < K::kBlocks; tp++) {
+                const bf16x8 v = as_bf16x8(*(const uint4*)(rin[tp] + mm * kStep * WI * 16));
+#pragma unroll
+                for (int h = 0; h < K::kHalves; h++)
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v, acc[h], 0, 0, 0);
+            }
+            const int iy = oy + y, ix = ox + x;
+            const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
+#pragma unroll
+            for (int h = 0; h < K::kHalves; h++)
+                store_block(o + h * pout + mm * kStep * PO * 16, acc[h], keep, y < WO);
+        }
+    }
+}
+
+// conv 1 -> C on the 22 x 22 a1 region at image pixel (i0-3, j0-3), from
+// the packed r0 tile s0, on v_mfma_f32_16x16x16_bf16 (the 12 products of a
+// 3 x 3 window and 4 zeros per pixel).  Groups of 16 pixels: the first 16
+// columns of each row (wave w: rows w, w + 4, ..., two at a time), then the
+// columns 16..21 as blocks of 2 rows x 8 columns (lanes of columns 22 and
+// 23 idle).  A lane's operand and store addresses are fixed; rows are
+// immediate offsets.
+template <int C, bool kEdge>
+__device__ __forceinline__ void conv_in(const uint32_t* s0, uint16_t* sa, const s16x4 (&w1)[C / 16],
+                                        const f32x4 (&bias1)[C / 16], int i0, int j0, int n, int wave, int j16,
+                                        int kb) {
+    constexpr int H = C / 16;
+    const uint32_t m = kb < 3 ? 0xffffffffu : 0u;
+    const int ry = min(kb, 2);  // tile row offset of this lane's operand
+    auto base = [&](int x) { return ((x & 1) ? s0 + kS0Odd : s0) + (x >> 1); };
+    {   // first 16 columns
+        const uint32_t* const p = base(j16) + (wave + ry) * kS0;
+        uint16_t* const o = sa + chunk_off(wave, j16, kF1, kb >> 1) + 4 * (kb & 1);
+        const bool xin = !kEdge || (j0 - 3 + j16 >= 0 && j0 - 3 + j16 < n);
+        // rows wave + 4r: r = 0..3, then 4 and (waves 0, 1) 5
+        auto rows = [&](auto nr_tag, int r0) {
+            constexpr int NR = decltype(nr_tag)::value;
+            s16x4 v[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) v[r] = in_operand(p, 4 * (r0 + r) * kS0, m);
+            f32x4 acc[NR][H];
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+#pragma unroll
+                for (int h = 0; h < H; h++)
+                    acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[h], v[r], bias1[h], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int y = i0 - 3 + wave + 4 * (r0 + r);
+                const bool keep = xin && (!kEdge || (y >= 0 && y < n));
+#pragma unroll
+                for (int h = 0; h < H; h++) store_block(o + h * kPlA + 4 * (r0 + r) * kF1 * 16, acc[r][h], keep, true);
+            }
+        };
+        rows(std::integral_constant<int, 4>(), 0);
+        if (wave + 20 < kF1)
+            rows(std::integral_constant<int, 2>(), 4);
+        else
+            rows(std::integral_constant<int, 1>(), 4);
+    }
+    {   // columns 16..21: group g covers rows 2g, 2g + 1
+        const int r = j16 >> 3, c = j16 & 7;
+        const bool cin = c < kF1 - 16;
+        const int x = cin ? 16 + c : kF1 - 1;
+        const uint32_t* const p = base(x) + (2 * wave + r + ry) * kS0;
+        uint16_t* const o = sa + chunk_off(2 * wave + r, x, kF1, kb >> 1) + 4 * (kb & 1);
+        const bool xin = !kEdge || (j0 - 3 + x >= 0 && j0 - 3 + x < n);
+#pragma unroll
+        for (int mm = 0; mm < (kF1 / 2 + kFWaves - 1) / kFWaves; mm++) {
+            if (wave + kFWaves * mm >= kF1 / 2) break;  // wave-uniform
+            const int y = 2 * wave + r + 8 * mm;
+            const s16x4 v = in_operand(p, 8 * mm * kS0, m);
+            const bool keep = xin && (!kEdge || (i0 - 3 + y >= 0 && i0 - 3 + y < n));
+            f32x4 acc[H];
+#pragma unroll
+            for (int h = 0; h < H; h++) acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[h], v, bias1[h], 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < H; h++) store_block(o + h * kPlA + 8 * mm * kF1 * 16, acc[h], keep, cin);
+        }
+    }
+}
+
+// Copy the centre 16 x 16 pixels of a planar LDS region (side wi, plane
+// stride pl, centre offset `off`) to the NHWC image `dst` (training: the
+// backward kernels read it).
 template <int C>
-__device__ __forceinline__ void store_centre(const uint16_t* reg, int wi, int off, uint16_t* __restrict__ dst,
-                                             int i0, int j0, int n) {
+__device__ __forceinline__ void store_centre(const uint16_t* reg, int pl, int wi, int off,
+                                             uint16_t* __restrict__ dst, int i0, int j0, int n) {
     constexpr int kV = C / 8;  // 16-byte vectors per pixel
     for (int t = threadIdx.x; t < kFT * kFT * kV; t += 64 * kFWaves) {
-        const int px = t / kV, h = t % kV;
+        const int px = t / kV, h = t % kV;  // h: channels 8h .. 8h+7 (plane h / 2, half h % 2)
         const int y = px / kFT, x = px % kFT;
-        const uint4 v = *(const uint4*)(reg + ((y + off) * wi + x + off) * C + 8 * h);
+        const uint4 v = pair_swap(*(const uint4*)(reg + (h >> 1) * pl + chunk_off(y + off, x + off, wi, h & 1)));
         *(uint4*)(dst + ((size_t)(i0 + y) * n + j0 + x) * C + 8 * h) = v;
     }
 }
@@ -257,7 +409,7 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
                                          uint16_t* __restrict__ a1g, uint16_t* __restrict__ a2g,
                                          uint16_t* __restrict__ a3g, const float* __restrict__ target,
                                          float* __restrict__ dout, float inv_count, double* __restrict__ loss,
-                                         uint16_t* sa, uint16_t* sb, uint32_t* s0e, uint32_t* s0o, uint32_t* w4,
+                                         uint16_t* sa, uint16_t* sb, uint32_t* s0, uint32_t* w4,
                                          float* red) {
     using K = MidK<C>;
     constexpr int H = K::kHalves;
@@ -265,16 +417,17 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar: wave-uniform tests branch
     const int j16 = lane & 15, kb = lane >> 4;  // A row / B column, and k block
     // prologue loads in flight together: W2 operands, W1, biases, W4, r0 tile
     bf16x8 w[H][K::kBlocks];
     load_mid_weights<C>(w, wb2, j16, kb);
-    bf16x8 w1[H];
+    s16x4 w1[H];
     f32x4 bias1[H], bias2[H], bias3[H];
 #pragma unroll
     for (int h = 0; h < H; h++) {
-        w1[h] = as_bf16x8(kb < 2 ? *(const uint4*)(w1b + (16 * h + j16) * 16 + 8 * kb) : make_uint4(0, 0, 0, 0));
+        w1[h] = __builtin_bit_cast(s16x4, *(const uint2*)(w1b + (16 * h + j16) * 16 + 4 * kb));  // W1[co][ky = kb][0..3]
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             bias1[h][r] = prm[L.B1 + 16 * h + 4 * kb + r];
@@ -282,115 +435,80 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
             bias3[h][r] = prm[L.B3 + 16 * h + 4 * kb + r];
         }
     }
-    {   // all of this thread's r0 loads in flight together
-        constexpr int kIt = (kF0 * kS0 + 64 * kFWaves - 1) / (64 * kFWaves);
-        float v[kIt][3];
-#pragma unroll
-        for (int it = 0; it < kIt; it++) {
-            const int t = threadIdx.x + 64 * kFWaves * it;
-            const int row = t / kS0, c = 2 * (t % kS0);  // columns c, c+1, c+2 of the tile (c+2 may be the pad 24)
-            const int ii = i0 - 4 + row;
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int jj = j0 - 4 + c + q;
-                v[it][q] = (t < kF0 * kS0 && c + q < kF0 && (!kEdge || (ii >= 0 && ii < n && jj >= 0 && jj < n)))
-                               ? img[(size_t)ii * n + jj]
-                               : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < kIt; it++) {
-            const int t = threadIdx.x + 64 * kFWaves * it;
-            if (t < kF0 * kS0) {
-                s0e[t] = pack_bf16(v[it][0], v[it][1]);
-                s0o[t] = pack_bf16(v[it][1], v[it][2]);
-            }
+    {   // the 24 x 24 r0 tile as 4-column quads: thread t < 144 loads quad
+        // t % 6 of tile row t / 6 (columns j0 - 4 + 4q .. +3: a multiple of 4,
+        // so a quad lies wholly inside or outside the image) and the next
+        // column, and writes its two even- and two odd-column bf16 pairs
+        constexpr int kQ = kF0 / 4;
+        const int t = threadIdx.x;
+        if (t < kF0 * kQ) {
+            const int row = t / kQ, q = t % kQ;
+            const int ii = i0 - 4 + row, jj = j0 - 4 + 4 * q;
+            const bool rin = !kEdge || (ii >= 0 && ii < n);
+            const float* src = img + (size_t)ii * n + jj;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float e = 0.0f;  // column 4q + 4 (the tile's pad column 24 for q = 5)
+            if (rin && (!kEdge || (jj >= 0 && jj < n))) v = *(const float4*)src;
+            if (rin && q + 1 < kQ && (!kEdge || (jj + 4 >= 0 && jj + 4 < n))) e = src[4];
+            *(uint2*)(s0 + row * kS0 + 2 * q) = make_uint2(pack_bf16(v.x, v.y), pack_bf16(v.z, v.w));
+            *(uint2*)(s0 + kS0Odd + row * kS0 + 2 * q) = make_uint2(pack_bf16(v.y, v.z), pack_bf16(v.w, e));
         }
     }
     for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves)
         w4[t] = pack_bf16(prm[L.W4 + 2 * t], prm[L.W4 + 2 * t + 1]);
     __syncthreads();
-    {   // a1 on the 22 x 22 region at (i0-3, j0-3)
-        const int count = kF1 * kF1;
-        for (int g0 = wave; g0 * 16 < count; g0 += 2 * kFWaves) {  // two groups at a time
-            const int g1 = g0 + kFWaves;
-            const int p0 = g0 * 16 + j16, p1 = g1 * 16 + j16;
-            const int q0 = min(p0, count - 1), q1 = min(p1, count - 1);
-            const int y0 = q0 / kF1, x0 = q0 % kF1, y1 = q1 / kF1, x1 = q1 % kF1;
-            const bf16x8 v0 = in_operand(s0e, s0o, y0, x0, kb), v1 = in_operand(s0e, s0o, y1, x1, kb);
-            int iy = i0 - 3 + y0, ix = j0 - 3 + x0;
-            const bool keep0 = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
-            iy = i0 - 3 + y1;
-            ix = j0 - 3 + x1;
-            const bool keep1 = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
-#pragma unroll
-            for (int h = 0; h < H; h++) {
-                const f32x4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[h], v0, bias1[h], 0, 0, 0);
-                const f32x4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[h], v1, bias1[h], 0, 0, 0);
-                if (p0 < count) store4(sa + p0 * C + 16 * h + 4 * kb, acc0, keep0);
-                if (p1 < count) store4(sa + p1 * C + 16 * h + 4 * kb, acc1, keep1);
-            }
-        }
-    }
+    conv_in<C, kEdge>(s0, sa, w1, bias1, i0, j0, n, wave, j16, kb);
     __syncthreads();
     const size_t img_off = (size_t)b * n * n * C;
-    if (kTrain) store_centre<C>(sa, kF1, 3, a1g + img_off, i0, j0, n);
-    mid_layer<C, kEdge>(sa, sb, kF1, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
+    if (kTrain) store_centre<C>(sa, kPlA, kF1, 3, a1g + img_off, i0, j0, n);
+    mid_layer<C, kEdge, kF1, kF2>(sa, kPlA, sb, kPlB, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
     load_mid_weights<C>(w, wb3, j16, kb);  // in flight while the other waves finish
     __syncthreads();
-    if (kTrain) store_centre<C>(sb, kF2, 2, a2g + img_off, i0, j0, n);
-    mid_layer<C, kEdge>(sb, sa, kF2, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
+    if (kTrain) store_centre<C>(sb, kPlB, kF2, 2, a2g + img_off, i0, j0, n);
+    mid_layer<C, kEdge, kF2, kP3>(sb, kPlB, sa, kPlA, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
-    if (kTrain) store_centre<C>(sa, kF3, 1, a3g + img_off, i0, j0, n);
-    // conv C -> 1 + bias + residual: one thread per horizontal pixel pair and
-    // channel half.  The pair's 3 x 4 neighbourhood is read once (12 reads
-    // of each 16-byte vector instead of 2 x 9); each pixel still sums its
-    // taps 0..8 in order.
-    constexpr int kQ = C / 16;  // 16-byte vectors per pixel and channel half
-    float dsum = 0.0f;          // training: this thread's squared errors
-    for (int t = threadIdx.x; t < kFT * kFT; t += 64 * kFWaves) {
-        const int pr = t >> 1, part = t & 1;
-        const int y = pr / (kFT / 2), x = 2 * (pr % (kFT / 2));
-        float acc0 = 0.0f, acc1 = 0.0f;
+    if (kTrain) store_centre<C>(sa, kPlA, kP3, 1, a3g + img_off, i0, j0, n);
+    // conv C -> 1 + bias + residual on the matrix cores: the K blocks and
+    // operand reads of the mid layers over the a3 region, with W4 in every
+    // row of A, so every row of D is the output; row 0 (lanes 0..15,
+    // register 0) is used.  Wave w takes output rows w, w + 4, w + 8, w + 12.
+    float dsum = 0.0f;  // training: this thread's squared errors
+    {
+        const int ch = kb % K::kChunks;
+        bf16x8 w4a[K::kBlocks];
+        const uint16_t* rin[K::kBlocks];
 #pragma unroll
-        for (int ty = 0; ty < 3; ty++) {
-            uint4 c[4][kQ];
+        for (int tp = 0; tp < K::kBlocks; tp++) {
+            const int tap = K::kTapsPer * tp + kb / K::kChunks, tc = min(tap, 8);
+            const uint4 wv = pair_swap(*(const uint4*)(w4 + tc * (C / 2) + 4 * ch));  // W4[tap][8ch .. 8ch+7]
+            w4a[tp] = as_bf16x8(tap < 9 ? wv : make_uint4(0, 0, 0, 0));
+            rin[tp] = sa + (ch >> 1) * kPlA + chunk_off(wave + tc / 3, j16 + tc % 3, kP3, ch & 1);
+        }
+        const float b4 = prm[L.B4];
+        f32x4 acc[4];  // rows wave + 4r
 #pragma unroll
-            for (int cx = 0; cx < 4; cx++)
+        for (int r = 0; r < 4; r++) acc[r] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int q = 0; q < kQ; q++)
-                    c[cx][q] = *(const uint4*)(sa + ((y + ty) * kF3 + x + cx) * C + (C / 2) * part + 8 * q);
+        for (int tp = 0; tp < K::kBlocks; tp++) {
+            bf16x8 v[4];
 #pragma unroll
-            for (int tx = 0; tx < 3; tx++) {
-                const uint32_t* wp = w4 + (3 * ty + tx) * (C / 2) + (C / 4) * part;
+            for (int r = 0; r < 4; r++) v[r] = as_bf16x8(*(const uint4*)(rin[tp] + 4 * r * kP3 * 16));
 #pragma unroll
-                for (int q = 0; q < kQ; q++) {
-                    acc0 = dot2_bf16(wp[4 * q + 0], c[tx][q].x, acc0);
-                    acc0 = dot2_bf16(wp[4 * q + 1], c[tx][q].y, acc0);
-                    acc0 = dot2_bf16(wp[4 * q + 2], c[tx][q].z, acc0);
-                    acc0 = dot2_bf16(wp[4 * q + 3], c[tx][q].w, acc0);
-                }
+            for (int r = 0; r < 4; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4a[tp], v[r], acc[r], 0, 0, 0);
+        }
+        if (kb == 0) {
 #pragma unroll
-                for (int q = 0; q < kQ; q++) {
-                    acc1 = dot2_bf16(wp[4 * q + 0], c[tx + 1][q].x, acc1);
-                    acc1 = dot2_bf16(wp[4 * q + 1], c[tx + 1][q].y, acc1);
-                    acc1 = dot2_bf16(wp[4 * q + 2], c[tx + 1][q].z, acc1);
-                    acc1 = dot2_bf16(wp[4 * q + 3], c[tx + 1][q].w, acc1);
+            for (int r = 0; r < 4; r++) {
+                const int y = wave + 4 * r;
+                const size_t p = ((size_t)b * n + i0 + y) * n + j0 + j16;
+                const float o = img[(size_t)(i0 + y) * n + j0 + j16] + (acc[r][0] + b4);
+                out[p] = o;
+                if (kTrain && target) {
+                    const float d = o - target[p];
+                    dout[p] = 2.0f * d * inv_count;
+                    dsum += d * d;
                 }
             }
-        }
-        // the other half's sums; the part-0 thread finishes pixel x, part 1 pixel x + 1
-        const float o0 = __shfl_xor(acc0, 1, 64), o1 = __shfl_xor(acc1, 1, 64);
-        const float s0v = part == 0 ? acc0 : o1;  // channel half 0 of this thread's pixel
-        const float s1v = part == 0 ? o0 : acc1;  // channel half 1
-        const int xx = x + part;
-        const size_t p = ((size_t)b * n + i0 + y) * n + j0 + xx;
-        const float o = img[(size_t)(i0 + y) * n + j0 + xx] + ((s0v + s1v) + prm[L.B4]);
-        out[p] = o;
-        if (kTrain && target) {
-            const float d = o - target[p];
-            dout[p] = 2.0f * d * inv_count;
-            dsum += d * d;
         }
     }
     if (kTrain && target) {  // uniform
@@ -405,8 +523,10 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
 }
 
 
+// C = 16: at most 96 VGPRs, so the five workgroups per CU that LDS allows
+// (five waves per SIMD) also fit the registers.
 template <int C, bool kTrain>
-__global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __restrict__ r0,
+__global__ __launch_bounds__(64 * kFWaves, C == 16 ? 5 : 2) void k_cnn_fused_fwd(const float* __restrict__ r0,
                                                                 const float* __restrict__ prm,
                                                                 const uint16_t* __restrict__ w1b,
                                                                 const uint16_t* __restrict__ wb2,
@@ -417,19 +537,19 @@ __global__ __launch_bounds__(64 * kFWaves) void k_cnn_fused_fwd(const float* __r
                                                                 const float* __restrict__ target,
                                                                 float* __restrict__ dout, float inv_count,
                                                                 double* __restrict__ loss) {
-    __shared__ __align__(16) uint16_t sa[kF1 * kF1 * C];  // a1, then a3
-    __shared__ __align__(16) uint16_t sb[kF2 * kF2 * C];  // a2
-    __shared__ uint32_t s0e[kF0 * kS0], s0o[kF0 * kS0];   // r0 as bf16 pairs at even / odd columns
-    __shared__ uint32_t w4[9 * C / 2];                    // W4 as bf16 pairs
+    __shared__ __align__(16) uint16_t sa[(C / 16) * kPlA];  // a1, then a3 (planar)
+    __shared__ __align__(16) uint16_t sb[(C / 16) * kPlB];  // a2 (planar)
+    __shared__ __align__(16) uint32_t s0[kS0Odd + kF0 * kS0];             // r0 as bf16 pairs at even, then odd columns
+    __shared__ __align__(16) uint32_t w4[9 * C / 2];      // W4 as bf16 pairs
     __shared__ float red[64 * kFWaves];
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const bool interior = i0 >= 4 && i0 + kFT + 4 <= n && j0 >= 4 && j0 + kFT + 4 <= n;  // uniform
     if (interior)
         cnn_tile<C, kTrain, false>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa,
-                                   sb, s0e, s0o, w4, red);
+                                   sb, s0, w4, red);
     else
         cnn_tile<C, kTrain, true>(r0, prm, w1b, wb2, wb3, out, n, a1g, a2g, a3g, target, dout, inv_count, loss, sa,
-                                  sb, s0e, s0o, w4, red);
+                                  sb, s0, w4, red);
 }
 
 // ---------------------------------------------------------------------------
