@@ -48,17 +48,25 @@ constexpr int T = 16;  // tile side
 // forward
 // ---------------------------------------------------------------------------
 
-// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout, and W1 as
-// bf16 [co][16] with k = 4*ky + kx (kx = 3 and k >= 12 zero): the
-// matrix-core operands.
+// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout and transposed
+// ([ci][t][co], the backward's), and W1 as bf16 [co][16] with k = 4*ky + kx
+// (kx = 3 and k >= 12 zero): the matrix-core operands.
 template <int C>
-__global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb2, uint16_t* __restrict__ wb3,
-                               uint16_t* __restrict__ w1b) {
+__global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb) {
     constexpr LearnedLayout L = learned_layout(C);
+    uint16_t* const wb2 = wb;
+    uint16_t* const wb3 = wb + 9 * C * C;
+    uint16_t* const w1b = wb + 2 * 9 * C * C;
+    uint16_t* const wt2 = w1b + 16 * C;
+    uint16_t* const wt3 = wt2 + 9 * C * C;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 9 * C * C) {
-        wb2[i] = f2bf(prm[L.W2 + i]);
-        wb3[i] = f2bf(prm[L.W3 + i]);
+        const int co = i / (9 * C), t = (i / C) % 9, ci = i % C;
+        const uint16_t v2 = f2bf(prm[L.W2 + i]), v3 = f2bf(prm[L.W3 + i]);
+        wb2[i] = v2;
+        wb3[i] = v3;
+        wt2[(ci * 9 + t) * C + co] = v2;
+        wt3[(ci * 9 + t) * C + co] = v3;
     }
     if (i < 16 * C) {
         const int co = i / 16, k = i % 16, ky = k / 4, kx = k % 4;
@@ -112,6 +120,7 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // spends 52% of its LDS cycles in bank conflicts, measured.)
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kFT = 16;          // output tile side
 constexpr int kF0 = kFT + 8;     // r0 region side
 constexpr int kF1 = kFT + 6;     // a1
@@ -585,47 +594,8 @@ __global__ __launch_bounds__(256) void k_conv_out_bwd(const float* __restrict__ 
     }
 }
 
-// dX[q][ci] = sum_{t,co} W[co][t][ci] * dZ[q - off(t)][co], times (x > 0).
-// 16 x 8 pixel tiles so that the fp32 dZ halo and the weights fit in LDS.
+// 16 x 8 pixel tiles for the single-channel layers' backward kernels.
 constexpr int TY = 8;
-template <int C>
-__global__ __launch_bounds__(128) void k_conv_mid_bwd(const float* __restrict__ dz, const uint16_t* __restrict__ x,
-                                                      const float* __restrict__ prm, int woff,
-                                                      float* __restrict__ dzp, int n) {
-    extern __shared__ float sm[];
-    float* w = sm;               // [t][co][ci]
-    float* ds = sm + 9 * C * C;  // (TY+2)*(T+2) pixels at a pitch of C+1 floats (distinct banks per lane)
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * T;
-    for (int t = threadIdx.x; t < 9 * C * C; t += 128) {
-        const int co = t / (9 * C), r = t % (9 * C), tap = r / C, ci = r % C;
-        w[(tap * C + co) * C + ci] = rbf(prm[woff + t]);
-    }
-    const float* db = dz + (size_t)b * n * n * C;
-    for (int t = threadIdx.x; t < (TY + 2) * (T + 2) * C; t += 128) {
-        const int pix = t / C, c = t % C;
-        const int ii = i0 + pix / (T + 2) - 1, jj = j0 + pix % (T + 2) - 1;
-        ds[pix * (C + 1) + c] = (ii >= 0 && ii < n && jj >= 0 && jj < n) ? db[((size_t)ii * n + jj) * C + c] : 0.0f;
-    }
-    __syncthreads();
-    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float acc[C];
-#pragma unroll
-    for (int ci = 0; ci < C; ci++) acc[ci] = 0.0f;
-    for (int tap = 0; tap < 9; tap++) {
-        const float* gp = ds + ((ty + 2 - tap / 3) * (T + 2) + tx + 2 - tap % 3) * (C + 1);
-        for (int co = 0; co < C; co++) {
-            const float gv = gp[co];
-            const float* wr = w + (tap * C + co) * C;
-#pragma unroll
-            for (int ci = 0; ci < C; ci++) acc[ci] = __builtin_fmaf(wr[ci], gv, acc[ci]);
-        }
-    }
-    const size_t p = ((size_t)b * n + i0 + ty) * n + j0 + tx;
-#pragma unroll
-    for (int ci = 0; ci < C; ci++) dzp[p * C + ci] = bf2f(x[p * C + ci]) > 0.0f ? acc[ci] : 0.0f;
-}
-
 // dr0[q] = dout[q] + sum_{t,co} W1[co][t] * dZ1[q - off(t)][co].
 template <int C>
 __global__ __launch_bounds__(128) void k_conv_in_bwd(const float* __restrict__ dz1, const float* __restrict__ dout,
@@ -728,6 +698,252 @@ __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ dz, int
         if (wi < nw) atomicAdd(grad + woff + wi, acc[r]);
     }
     if ((int)threadIdx.x < zc) atomicAdd(grad + boff + threadIdx.x, bacc);
+}
+
+// ---------------------------------------------------------------------------
+// backward of the C -> C layers on the matrix cores
+// ---------------------------------------------------------------------------
+
+// fp32 pair -> bf16 hi and lo parts (hi = bf16(v), lo = bf16(v - hi); v - hi
+// is exact in fp32), packed as pairs: hi + lo equals v to 2^-17 of |v|, so an
+// MFMA on each part keeps the product with a bf16 operand to that accuracy.
+__device__ __forceinline__ void split_bf16(float a, float b, uint32_t& hi, uint32_t& lo) {
+    hi = pack_bf16(a, b);
+    lo = pack_bf16(a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u));
+}
+
+// Activation gradient of a conv C -> C layer:
+//   dzp[q][ci] = (x[q][ci] > 0) * sum_{t,co} W[co][t][ci] dz[q - off(t)][co],
+// off(t) = (t/3 - 1, t%3 - 1).  One workgroup per 16 x 16 tile.  The fp32
+// dz of the tile and a one-pixel halo goes to two planar bf16 LDS regions,
+// its hi and lo parts (split_bf16), laid out like the forward's (chunk_off,
+// channels in order); each K block runs one v_mfma_f32_16x16x32_bf16 on
+// each part with fp32 accumulation.  A: the transposed weights wt[ci][t][co].
+// K blocks as in the forward (tap pairs at C = 16, single taps at C = 32);
+// tap t reads halo pixel (y + 2 - t/3, x + 2 - t%3).  Wave w takes rows w,
+// w + 4, w + 8, w + 12, four accumulator chains.
+template <int C>
+__global__ __launch_bounds__(256) void k_conv_mid_bwd_mfma(const float* __restrict__ dz,
+                                                           const uint16_t* __restrict__ x,
+                                                           const uint16_t* __restrict__ wt,
+                                                           float* __restrict__ dzp, int n) {
+    using K = MidK<C>;
+    constexpr int H = K::kHalves;
+    constexpr int kW = kFT + 2;         // halo tile side
+    constexpr int kPl = kW * kW * 16;   // plane stride (elements)
+    __shared__ __align__(16) uint16_t sh[(C / 16) * kPl];
+    __shared__ __align__(16) uint16_t sl[(C / 16) * kPl];
+    const int b = blockIdx.z, i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j16 = lane & 15, kb = lane >> 4;
+    bf16x8 w[H][K::kBlocks];
+#pragma unroll
+    for (int h = 0; h < H; h++)
+#pragma unroll
+        for (int tp = 0; tp < K::kBlocks; tp++) {
+            const int tap = K::kTapsPer * tp + kb / K::kChunks;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tap < 9) v = *(const uint4*)(wt + ((16 * h + j16) * 9 + tap) * C + 8 * (kb % K::kChunks));
+            w[h][tp] = as_bf16x8(v);
+        }
+    const float* db = dz + (size_t)b * n * n * C;
+    for (int t = threadIdx.x; t < kW * kW * (C / 8); t += 64 * kFWaves) {
+        const int px = t / (C / 8), c = t % (C / 8);  // halo pixel and 8-channel chunk
+        const int y = px / kW, xx = px % kW;
+        const int ii = i0 - 1 + y, jj = j0 - 1 + xx;
+        float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0;
+        if (ii >= 0 && ii < n && jj >= 0 && jj < n) {
+            const float4* src = (const float4*)(db + ((size_t)ii * n + jj) * C + 8 * c);
+            v0 = src[0];
+            v1 = src[1];
+        }
+        uint32_t hi[4], lo[4];
+        split_bf16(v0.x, v0.y, hi[0], lo[0]);
+        split_bf16(v0.z, v0.w, hi[1], lo[1]);
+        split_bf16(v1.x, v1.y, hi[2], lo[2]);
+        split_bf16(v1.z, v1.w, hi[3], lo[3]);
+        const int off = (c >> 1) * kPl + chunk_off(y, xx, kW, c & 1);
+        *(u32x4*)(sh + off) = (u32x4){hi[0], hi[1], hi[2], hi[3]};
+        *(u32x4*)(sl + off) = (u32x4){lo[0], lo[1], lo[2], lo[3]};
+    }
+    __syncthreads();
+    const int ch = kb % K::kChunks;
+    const uint16_t* rh[K::kBlocks];
+    const uint16_t* rl[K::kBlocks];
+#pragma unroll
+    for (int tp = 0; tp < K::kBlocks; tp++) {
+        const int tap = min(K::kTapsPer * tp + kb / K::kChunks, 8);
+        const int o = (ch >> 1) * kPl + chunk_off(wave + 2 - tap / 3, j16 + 2 - tap % 3, kW, ch & 1);
+        rh[tp] = sh + o;
+        rl[tp] = sl + o;
+    }
+    f32x4 acc[4][H];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int h = 0; h < H; h++) acc[r][h] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tp = 0; tp < K::kBlocks; tp++) {
+        bf16x8 vh[4], vl[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            vh[r] = as_bf16x8(*(const uint4*)(rh[tp] + 4 * r * kW * 16));
+            vl[r] = as_bf16x8(*(const uint4*)(rl[tp] + 4 * r * kW * 16));
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+                acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], vh[r], acc[r][h], 0, 0, 0);
+                acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], vl[r], acc[r][h], 0, 0, 0);
+            }
+    }
+    // lane (j16, kb) holds channels 16h + 4kb .. +3 of pixel (wave + 4r, j16)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const size_t q = ((size_t)b * n + i0 + wave + 4 * r) * n + j0 + j16;
+#pragma unroll
+        for (int h = 0; h < H; h++) {
+            const uint2 xv = *(const uint2*)(x + q * C + 16 * h + 4 * kb);
+            float4 o;
+            o.x = __uint_as_float(xv.x << 16) > 0.0f ? acc[r][h][0] : 0.0f;
+            o.y = __uint_as_float(xv.x & 0xffff0000u) > 0.0f ? acc[r][h][1] : 0.0f;
+            o.z = __uint_as_float(xv.y << 16) > 0.0f ? acc[r][h][2] : 0.0f;
+            o.w = __uint_as_float(xv.y & 0xffff0000u) > 0.0f ? acc[r][h][3] : 0.0f;
+            *(float4*)(dzp + q * C + 16 * h + 4 * kb) = o;
+        }
+    }
+}
+
+// Weight gradient of a conv C -> C layer:
+//   dW[co][t][ci] += sum_p dz[p][co] x[p + off(t)][ci],  db[co] += sum_p dz[p][co].
+// One workgroup per 16 x 16 tile, summed over the slices b = blockIdx.z,
+// blockIdx.z + gridDim.z, ... in registers, then added into grad[] with
+// atomics once.  K = pixels: A[co][p] = dz, split into bf16 hi and lo parts
+// (split_bf16) and transposed in LDS ([co][16 rows][16 columns], co stride
+// 272 elements); B[p][ci] = x (bf16, exact), transposed with a one-pixel
+// halo ([ci][18 rows][pitch 24]).  A K block is two tile rows: lane block kb
+// holds row kb >> 1, columns 8 (kb & 1) .. +7.  Tap (ty, tx) shifts B by ty
+// rows and tx columns; the column shift comes from one aligned 16-byte read,
+// the next 4 bytes and v_alignbyte.  The co stride of A and the ci stride of
+// B keep each 16-byte read cycle on distinct banks.  Wave w takes the taps
+// w, w + 4 and (w = 0) 8.
+template <int C>
+__global__ __launch_bounds__(256) void k_wgrad_mfma(const float* __restrict__ dz, const uint16_t* __restrict__ x,
+                                                    float* __restrict__ grad, int woff, int boff, int n, int batch) {
+    constexpr int H = C / 16;
+    constexpr int kAS = 272;         // A: co stride (elements)
+    constexpr int kXP = 24;          // B: row pitch (elements)
+    constexpr int kXS = 18 * kXP;    // B: ci stride
+    __shared__ __align__(16) uint16_t ah[C * kAS];
+    __shared__ __align__(16) uint16_t al[C * kAS];
+    __shared__ __align__(16) uint16_t xs[C * kXS];
+    __shared__ float bsum[C];
+    const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j16 = lane & 15, kb = lane >> 4;
+    if ((int)threadIdx.x < C) bsum[threadIdx.x] = 0.0f;
+    f32x4 acc[3][H][H];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int a = 0; a < H; a++)
+#pragma unroll
+            for (int c = 0; c < H; c++) acc[k][a][c] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    for (int b = blockIdx.z; b < batch; b += gridDim.z) {
+        __syncthreads();
+        // A: item (co, 8 pixels of a row)
+        for (int t = threadIdx.x; t < C * 32; t += 64 * kFWaves) {
+            const int co = t % C, pg = t / C;
+            const int row = pg >> 1, c0 = 8 * (pg & 1);
+            const float* src = dz + (((size_t)b * n + i0 + row) * n + j0 + c0) * C + co;
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = src[(size_t)e * C];
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) split_bf16(v[2 * e], v[2 * e + 1], hi[e], lo[e]);
+            *(u32x4*)(ah + co * kAS + row * 16 + c0) = (u32x4){hi[0], hi[1], hi[2], hi[3]};
+            *(u32x4*)(al + co * kAS + row * 16 + c0) = (u32x4){lo[0], lo[1], lo[2], lo[3]};
+            atomicAdd(&bsum[co], ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])));
+        }
+        // B: item (ci, halo row, 8 columns); columns 18..23 are zeros
+        for (int t = threadIdx.x; t < C * 18 * 3; t += 64 * kFWaves) {
+            const int ci = t % C, rg = t / C;
+            const int row = rg / 3, g = rg % 3;
+            const int ii = i0 - 1 + row;
+            uint32_t v[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                uint32_t pr = 0;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int col = 8 * g + 2 * e + u, jj = j0 - 1 + col;
+                    uint32_t h = 0;
+                    if (col < 18 && ii >= 0 && ii < n && jj >= 0 && jj < n)
+                        h = x[(((size_t)b * n + ii) * n + jj) * C + ci];
+                    pr |= h << (16 * u);
+                }
+                v[e] = pr;
+            }
+            *(u32x4*)(xs + ci * kXS + row * kXP + 8 * g) = (u32x4){v[0], v[1], v[2], v[3]};
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int kblk = 0; kblk < 8; kblk++) {
+            const int prow = 2 * kblk + (kb >> 1), pc = 8 * (kb & 1);
+            bf16x8 a_h[H], a_l[H];
+#pragma unroll
+            for (int a = 0; a < H; a++) {
+                a_h[a] = as_bf16x8(*(const uint4*)(ah + (16 * a + j16) * kAS + prow * 16 + pc));
+                a_l[a] = as_bf16x8(*(const uint4*)(al + (16 * a + j16) * kAS + prow * 16 + pc));
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int tap = wave + 4 * k;
+                if (tap > 8) break;  // wave-uniform
+                const int ty = tap / 3, tx = tap % 3;
+#pragma unroll
+                for (int c = 0; c < H; c++) {
+                    const uint16_t* xr = xs + (16 * c + j16) * kXS + (prow + ty) * kXP + pc;
+                    const uint4 d = *(const uint4*)xr;
+                    const uint32_t d4 = *(const uint32_t*)(xr + 8);
+                    uint4 s = d;  // columns pc + tx .. +7 of the row
+                    if (tx == 1) {
+                        s.x = __builtin_amdgcn_alignbyte(d.y, d.x, 2);
+                        s.y = __builtin_amdgcn_alignbyte(d.z, d.y, 2);
+                        s.z = __builtin_amdgcn_alignbyte(d.w, d.z, 2);
+                        s.w = __builtin_amdgcn_alignbyte(d4, d.w, 2);
+                    } else if (tx == 2) {
+                        s = make_uint4(d.y, d.z, d.w, d4);
+                    }
+                    const bf16x8 bv = as_bf16x8(s);
+#pragma unroll
+                    for (int a = 0; a < H; a++) {
+                        acc[k][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[a], bv, acc[k][a][c], 0, 0, 0);
+                        acc[k][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[a], bv, acc[k][a][c], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    // lane (j16, kb) holds D[co = 16a + 4kb + r][ci = 16c + j16]
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int tap = wave + 4 * k;
+        if (tap > 8) break;
+#pragma unroll
+        for (int a = 0; a < H; a++)
+#pragma unroll
+            for (int c = 0; c < H; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    atomicAdd(grad + woff + ((16 * a + 4 * kb + r) * 9 + tap) * C + 16 * c + j16, acc[k][a][c][r]);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < C) atomicAdd(grad + boff + threadIdx.x, bsum[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -883,13 +1099,13 @@ __global__ void k_noise(float* __restrict__ sino, size_t count, float i0, float 
 #define SKE_GRID(n, b) dim3((n) / T, (n) / T, (b))
 
 template <int C>
-hipError_t pack_weights_t(const float* prm, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_weights<C>, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb2, wb3, w1b);
+hipError_t pack_weights_t(const float* prm, uint16_t* wb, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_weights<C>, dim3((9 * C * C + 255) / 256), dim3(256), 0, s, prm, wb);
     return hipGetLastError();
 }
 
-hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s) {
-    return C == 32 ? pack_weights_t<32>(prm, wb2, wb3, w1b, s) : pack_weights_t<16>(prm, wb2, wb3, w1b, s);
+hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb, hipStream_t s) {
+    return C == 32 ? pack_weights_t<32>(prm, wb, s) : pack_weights_t<16>(prm, wb, s);
 }
 
 template <int C, bool kTrain>
@@ -922,10 +1138,10 @@ hipError_t backward_t(const LearnedBufs& B, const float* prm, const float* dout,
     constexpr LearnedLayout L = learned_layout(C);
     // activation gradients
     hipLaunchKernelGGL(k_conv_out_bwd<C>, SKE_GRID(n, batch), dim3(256), 0, s, dout, B.a3, prm, B.dz3, n);
-    const size_t lmid = sizeof(float) * (9 * C * C + (TY + 2) * (T + 2) * (C + 1));
+    const dim3 g16(n / kFT, n / kFT, batch);
+    hipLaunchKernelGGL(k_conv_mid_bwd_mfma<C>, g16, dim3(64 * kFWaves), 0, s, B.dz3, B.a2, B.wt3, B.dz2, n);
+    hipLaunchKernelGGL(k_conv_mid_bwd_mfma<C>, g16, dim3(64 * kFWaves), 0, s, B.dz2, B.a1, B.wt2, B.dz1, n);
     dim3 g8(n / T, n / TY, batch);
-    hipLaunchKernelGGL(k_conv_mid_bwd<C>, g8, dim3(128), lmid, s, B.dz3, B.a2, prm, L.W3, B.dz2, n);
-    hipLaunchKernelGGL(k_conv_mid_bwd<C>, g8, dim3(128), lmid, s, B.dz2, B.a1, prm, L.W2, B.dz1, n);
     hipLaunchKernelGGL(k_conv_in_bwd<C>, g8, dim3(128), 0, s, B.dz1, dout, prm, B.dr0, n);
     // weight gradients: about 2048 workgroups, each summing over a share of
     // the slices
@@ -934,11 +1150,10 @@ hipError_t backward_t(const LearnedBufs& B, const float* prm, const float* dout,
     const size_t lw4 = sizeof(float) * (TY * T * 1 + (TY + 2) * (T + 2) * C);
     hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lw4, s, dout, 1, B.a3, (const float*)nullptr, C, grad, L.W4, L.B4, 1,
                        n, batch);
-    const size_t lwm = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * C);
-    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lwm, s, B.dz3, C, B.a2, (const float*)nullptr, C, grad, L.W3, L.B3, 0,
-                       n, batch);
-    hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lwm, s, B.dz2, C, B.a1, (const float*)nullptr, C, grad, L.W2, L.B2, 0,
-                       n, batch);
+    const int tiles16 = (n / kFT) * (n / kFT);
+    const dim3 gm(n / kFT, n / kFT, (unsigned)std::max(1, std::min(batch, 2048 / tiles16)));
+    hipLaunchKernelGGL(k_wgrad_mfma<C>, gm, dim3(64 * kFWaves), 0, s, B.dz3, B.a2, grad, L.W3, L.B3, n, batch);
+    hipLaunchKernelGGL(k_wgrad_mfma<C>, gm, dim3(64 * kFWaves), 0, s, B.dz2, B.a1, grad, L.W2, L.B2, n, batch);
     const size_t lw1 = sizeof(float) * (TY * T * C + (TY + 2) * (T + 2) * 1);
     hipLaunchKernelGGL(k_wgrad<C>, gw, dim3(256), lw1, s, B.dz1, C, (const uint16_t*)nullptr, B.r0, 1, grad, L.W1,
                        L.B1, 0, n, batch);

@@ -28,7 +28,7 @@ struct ske_learned {
     DevBuf<float> prm, grad, m1, m2, full, dq, dfull, dout;
     DevBuf<double> loss;
     DevBuf<float> r0, dz, dr0;     // dz: three layers back to back
-    DevBuf<uint16_t> act, wb;      // act: a1, a2, a3 back to back; wb: bf16 W2, W3
+    DevBuf<uint16_t> act, wb;      // act: a1, a2, a3 back to back; wb: bf16 weights (learned_wb_size)
     DevBuf<float> ell;
     size_t pix(int batch) const { return (size_t)f->g.n * f->g.n * (size_t)batch; }
     LearnedBufs bufs(int batch, bool train) const {
@@ -49,6 +49,8 @@ struct ske_learned {
         B.wb2 = wb.p;
         B.wb3 = wb.p + 9 * C * C;
         B.w1b = wb.p + 2 * 9 * C * C;
+        B.wt2 = wb.p + 2 * 9 * C * C + 16 * C;
+        B.wt3 = wb.p + 3 * 9 * C * C + 16 * C;
         (void)batch;
         return B;
     }
@@ -103,7 +105,7 @@ int forward(ske_learned* l, int batch, bool train) {
             hipError_t e = launch_taps_expand(l->prm.p + l->L.Taps, l->full.p, g.ndet, s);
             if (e != hipSuccess) return e;
             const int C = l->C;
-            return launch_pack_weights(l->prm.p, C, l->wb.p, l->wb.p + 9 * C * C, l->wb.p + 2 * 9 * C * C, s);
+            return launch_pack_weights(l->prm.p, C, l->wb.p, s);
         }));
         l->derived = true;
     }
@@ -190,7 +192,7 @@ int ske_learned_create_channels(ske_fbp* f, int channels, ske_learned** out) {
     SKE_TRY(l->prm.ensure(l->nparam));
     SKE_TRY(l->r0.ensure(pix));
     SKE_TRY(l->act.ensure(3 * pix * channels));
-    SKE_TRY(l->wb.ensure(2 * 9 * channels * channels + 16 * channels));
+    SKE_TRY(l->wb.ensure(learned_wb_size(channels)));
     SKE_TRY(l->full.ensure(2 * (size_t)f->g.ndet - 1));
     SKE_TRY(ske_learned_init(l.get(), 1));
     *out = l.release();

@@ -43,11 +43,16 @@ struct LearnedBufs {
     float* dr0;          // fp32 gradient w.r.t. r0
     const uint16_t* wb2; // bf16 copies of W2, W3 ([co][t][ci])
     const uint16_t* wb3;
+    const uint16_t* wt2; // bf16 W2, W3 transposed ([ci][t][co]): the backward's operands
+    const uint16_t* wt3;
     const uint16_t* w1b; // bf16 W1 as [co][16] (9 taps, 7 zeros)
     int C;               // hidden channels (16 or 32)
 };
 
-hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb2, uint16_t* wb3, uint16_t* w1b, hipStream_t s);
+// wb: bf16 W2, W3 ([co][t][ci]), W1 as [co][16], then W2, W3 transposed
+// ([ci][t][co]); learned_wb_size(C) elements.
+constexpr size_t learned_wb_size(int C) { return 4 * 9 * (size_t)C * C + 16 * (size_t)C; }
+hipError_t launch_pack_weights(const float* prm, int C, uint16_t* wb, hipStream_t s);
 // Training forward: a1..a3 stored (n a multiple of 16).
 hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int batch, const float* target,
                               float* dout, double* loss, hipStream_t s);
