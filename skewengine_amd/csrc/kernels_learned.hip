@@ -94,9 +94,11 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 //   zeros: one MFMA per group and channel block; the operand is read as
 //   bf16 pairs (x, x+1) and (x+2, x+3) from two copies of the r0 tile
 //   packed at even and at odd columns;
-//   conv C -> C: K blocks of 32 = (32/C) taps x C channels, in tap order:
-//   the tap pairs (0,1), (2,3), (4,5), (6,7), (8,-) for C = 16, the nine
-//   taps for C = 32.
+//   conv C -> C: K blocks of 32 = (32/C) taps x C channels, arranged so that
+//   one operand read of an input row serves the three output rows above it
+//   (Sweep): for C = 16 the tap pairs ((dy,0), (dy,1)) for dy = 0, 1, 2 and
+//   ((0,2), (1,2)), ((1,2) zero, (2,2)); for C = 32 the nine taps.  A wave
+//   sweeps down a band of rows, reading each input row once.
 //   conv C -> 1: the K blocks of conv C -> C with W4 in every row of A
 //   (every row of D is the output), then + b4.
 // Activation tiles are bf16 in LDS, one plane per 16 channels: each region
@@ -110,7 +112,8 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // lanes' cycles included); ds_read_b32 two cycles of 32 lanes on 32 banks.
 // With this layout:
 //  - a layer's groups are the first 16 pixels of a row (then blocks of the
-//    leftover columns), whose operand reads meet no bank twice;
+//    leftover columns), whose operand reads meet no bank twice (the lanes of
+//    a read cycle lie in one row at one column shift);
 //  - every lane stores 8 bytes of a chunk after one v_permlane16_swap
 //    (store_block); a store cycle's 16 lanes meet each bank at most twice
 //    (the minimum for 16 pixels' same 8 bytes at a 32-byte pitch);
@@ -143,6 +146,36 @@ struct MidK {
     static constexpr int kChunks = C / 8;                            // 8-channel chunks per tap
     static constexpr int kBlocks = (9 + kTapsPer - 1) / kTapsPer;   // K blocks per layer (5 or 9)
     static constexpr int kHalves = C / 16;                           // 16-channel output blocks
+};
+
+// The K blocks as reads of a sweep down the rows.  Block b of output row y
+// multiplies the operand of "kind" kind(b) read at input row y + dy(b); lane
+// block kb of a kind-k read at row r takes the pixel (r + krow(k, kb),
+// j + kcol(k, kb)) of its column j.  C = 16: kind 0 = the taps (dy, 0) and
+// (dy, 1) of one row (pixel (r, j + kb/2)), kind 1 = the taps (dy, 2) of two
+// rows (pixel (r + kb/2, j + 2)); blocks 0..2 = kind 0 at dy = 0, 1, 2,
+// blocks 3, 4 = kind 1 at dy = 0, 1, where block 4's first half (tap (1, 2)
+// again) has zero weights.  C = 32: kind dx = the tap (dy, dx) of one row,
+// block 3 dy + dx.  A read of input row r thus serves output rows r, r - 1
+// and r - 2, and every block's lane taps are those of an ordinary 3 x 3
+// window: tap(b, kb) = 3 (dy(b) + krow) + kcol.
+template <int C>
+struct Sweep {
+    static constexpr int kKinds = C == 16 ? 2 : 3;
+    static constexpr int kind(int b) { return C == 16 ? (b < 3 ? 0 : 1) : b % 3; }
+    static constexpr int dy(int b) { return C == 16 ? (b < 3 ? b : b - 3) : b / 3; }
+    // the i-th block added to an output row by the sweep (by dy, then kind):
+    // groups outside the sweep add theirs in the same order, so a pixel's
+    // value does not depend on the group that computes it
+    static constexpr int order(int i) { return C == 16 ? (i & 1 ? 3 + (i >> 1) : i >> 1) : i; }
+    static __device__ __forceinline__ int krow(int k, int kb) { return (C == 16 && k == 1) ? (kb >> 1) : 0; }
+    static __device__ __forceinline__ int kcol(int k, int kb) { return C == 16 ? (k == 0 ? (kb >> 1) : 2) : k; }
+    // row and column of the lane's tap in block b; `zero`: the half without weights
+    static __device__ __forceinline__ void tap(int b, int kb, int& ty, int& tx, bool& zero) {
+        ty = dy(b) + krow(kind(b), kb);
+        tx = kcol(kind(b), kb);
+        zero = C == 16 && b == 4 && kb < 2;
+    }
 };
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -198,8 +231,9 @@ __device__ __forceinline__ void store_block(uint16_t* dst, const f32x4& acc, boo
 __device__ __forceinline__ uint4 pair_swap(uint4 v) { return make_uint4(v.x, v.z, v.y, v.w); }
 
 // A operands of a conv C -> C layer: lane (co = l%16, kb = l/16) of channel
-// block h holds W[16h + co][tap][ch .. ch+7] for K block tp, with
-// tap = (32/C)*tp + kb/(C/8) and ch = 8*(kb % (C/8)) (zero for tap 9).
+// block h holds W[16h + co][tap][ch .. ch+7] for K block b, with the lane's
+// tap of Sweep<C> and ch = 8*(kb % (C/8)) (zeros in block 4's first half
+// for C = 16).
 template <int C>
 __device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
                                                  const uint16_t* __restrict__ wb, int co, int kb) {
@@ -207,11 +241,13 @@ __device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[MidK<C>::kHalves][M
 #pragma unroll
     for (int h = 0; h < K::kHalves; h++)
 #pragma unroll
-        for (int tp = 0; tp < K::kBlocks; tp++) {
-            const int tap = K::kTapsPer * tp + kb / K::kChunks;
+        for (int b = 0; b < K::kBlocks; b++) {
+            int ty, tx;
+            bool zero;
+            Sweep<C>::tap(b, kb, ty, tx, zero);
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (tap < 9) v = pair_swap(*(const uint4*)(wb + ((16 * h + co) * 9 + tap) * C + 8 * (kb % K::kChunks)));
-            w[h][tp] = as_bf16x8(v);
+            if (!zero) v = pair_swap(*(const uint4*)(wb + ((16 * h + co) * 9 + 3 * ty + tx) * C + 8 * (kb % K::kChunks)));
+            w[h][b] = as_bf16x8(v);
         }
 }
 
@@ -225,72 +261,78 @@ struct MidLeft {
     static constexpr int kRows = 16 / kK;              // rows per group
     static constexpr int kGroups = (WO + kRows - 1) / kRows;
 };
-// NR rows of a mid layer's first 16 columns at once (NR independent
-// accumulator chains): rows 4 (K0 + r), r < NR, after the lane's own, of
-// the input at `rin` (per tap) and the output at `rout`; y0 is the image row
-// of the lane's first output row.
-template <int C, bool kEdge, int WI, int PO, int NR, int K0>
-__device__ __forceinline__ void mid_rows(const uint16_t* const (&rin)[MidK<C>::kBlocks], uint16_t* rout, int pout,
-                                         const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
-                                         const f32x4 (&bias)[MidK<C>::kHalves], int y0, int n, bool xin) {
+// R output rows of a mid layer's first 16 columns, swept downwards: each
+// input row r = 0 .. R+1 of the band is read once per kind (lane bases
+// `kin` at band row 0, rows as immediate offsets) and multiplies into the
+// output rows r, r - 1, r - 2 it feeds; output row y is complete, and
+// stored, once row y + 2 has been read.  yimg: the image row of band row 0.
+template <int C, bool kEdge, int WI, int PO, int R>
+__device__ __forceinline__ void mid_sweep(const uint16_t* const (&kin)[Sweep<C>::kKinds], uint16_t* rout, int pout,
+                                          const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
+                                          const f32x4 (&bias)[MidK<C>::kHalves], int yimg, int n, bool xin) {
     using K = MidK<C>;
-    f32x4 acc[NR][K::kHalves];
+    using S = Sweep<C>;
+    f32x4 acc[R][K::kHalves];
 #pragma unroll
-    for (int r = 0; r < NR; r++)
+    for (int y = 0; y < R; y++)
 #pragma unroll
-        for (int h = 0; h < K::kHalves; h++) acc[r][h] = bias[h];
+        for (int h = 0; h < K::kHalves; h++) acc[y][h] = bias[h];
 #pragma unroll
-    for (int tp = 0; tp < K::kBlocks; tp++) {
-        bf16x8 v[NR];
+    for (int r = 0; r < R + 2; r++) {
 #pragma unroll
-        for (int r = 0; r < NR; r++) v[r] = as_bf16x8(*(const uint4*)(rin[tp] + 4 * (K0 + r) * WI * 16));
+        for (int k = 0; k < S::kKinds; k++) {
+            bool need = false;
 #pragma unroll
-        for (int r = 0; r < NR; r++)
+            for (int b = 0; b < K::kBlocks; b++)
+                if (S::kind(b) == k && r - S::dy(b) >= 0 && r - S::dy(b) < R) need = true;
+            if (!need) continue;
+            const bf16x8 v = as_bf16x8(*(const uint4*)(kin[k] + r * WI * 16));
 #pragma unroll
-            for (int h = 0; h < K::kHalves; h++)
-                acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v[r], acc[r][h], 0, 0, 0);
-    }
+            for (int b = 0; b < K::kBlocks; b++) {
+                const int y = r - S::dy(b);
+                if (S::kind(b) != k || y < 0 || y >= R) continue;
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int y = y0 + 4 * (K0 + r);
-        const bool keep = xin && (!kEdge || (y >= 0 && y < n));
+                for (int h = 0; h < K::kHalves; h++)
+                    acc[y][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][b], v, acc[y][h], 0, 0, 0);
+            }
+        }
+        if (r >= 2) {
+            const int y = r - 2;
+            const bool keep = xin && (!kEdge || (yimg + y >= 0 && yimg + y < n));
 #pragma unroll
-        for (int h = 0; h < K::kHalves; h++) store_block(rout + h * pout + 4 * (K0 + r) * PO * 16, acc[r][h], keep, true);
+            for (int h = 0; h < K::kHalves; h++) store_block(rout + h * pout + y * PO * 16, acc[y][h], keep, true);
+        }
     }
 }
 
 // One conv C -> C layer from the LDS region `in` (side and row pitch WI,
 // plane stride pin) to `out` (side WI - 2, row pitch PO, plane stride pout)
 // whose top-left pixel is image pixel (oy, ox).  Groups of 16 pixels: the
-// first 16 columns of each row -- wave w takes rows w, w + 4, w + 8, ...,
-// two at a time (two independent accumulators per channel block), with its
-// operand and store addresses fixed per lane and the rows as immediate
-// offsets -- then the leftover columns.  Lanes of an absent tap 9 read tap 8
-// again; their weights are zero.
+// first 16 columns of each row -- wave w sweeps a band of consecutive rows
+// (side 20: five rows each; side 18: four, four, five, five) -- then the
+// leftover columns, whose groups read every block's operand themselves.
 template <int C, bool kEdge, int WI, int PO>
 __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t* out, int pout, int oy, int ox,
                                           int n, const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
                                           const f32x4 (&bias)[MidK<C>::kHalves], int wave, int j16, int kb) {
     using K = MidK<C>;
+    using S = Sweep<C>;
     constexpr int WO = WI - 2;
+    static_assert(WO == 20 || WO == 18, "mid layer sides 20 and 18");
     const int ch = kb % K::kChunks;  // this lane's 8-channel chunk of every tap
-    int tpy[K::kBlocks], tpx[K::kBlocks];
-#pragma unroll
-    for (int tp = 0; tp < K::kBlocks; tp++) {
-        const int tap = min(K::kTapsPer * tp + kb / K::kChunks, 8);
-        tpy[tp] = tap / 3;
-        tpx[tp] = tap % 3;
-    }
     const uint16_t* const cin = in + (ch >> 1) * pin;
     {
-        const uint16_t* rin[K::kBlocks];  // row `wave` of this lane's taps
+        const bool five = WO == 20 || wave >= 2;  // wave-uniform
+        const int y0 = WO == 20 ? 5 * wave : (wave < 2 ? 4 * wave : 8 + 5 * (wave - 2));
+        const uint16_t* kin[S::kKinds];
 #pragma unroll
-        for (int tp = 0; tp < K::kBlocks; tp++) rin[tp] = cin + chunk_off(wave + tpy[tp], j16 + tpx[tp], WI, ch);
-        uint16_t* const rout = out + chunk_off(wave, j16, PO, kb >> 1) + 4 * (kb & 1);
+        for (int k = 0; k < S::kKinds; k++) kin[k] = cin + chunk_off(y0 + S::krow(k, kb), j16 + S::kcol(k, kb), WI, ch);
+        uint16_t* const rout = out + chunk_off(y0, j16, PO, kb >> 1) + 4 * (kb & 1);
         const bool xin = !kEdge || (ox + j16 >= 0 && ox + j16 < n);
-        // rows wave, wave + 4, wave + 8, wave + 12 (inside any layer of side >= 16), then wave + 16
-        mid_rows<C, kEdge, WI, PO, 4, 0>(rin, rout, pout, w, bias, oy + wave, n, xin);
-        if (wave + 16 < WO) mid_rows<C, kEdge, WI, PO, 1, 4>(rin, rout, pout, w, bias, oy + wave, n, xin);
+        if (five)
+            mid_sweep<C, kEdge, WI, PO, 5>(kin, rout, pout, w, bias, oy + y0, n, xin);
+        else
+            mid_sweep<C, kEdge, WI, PO, 4>(kin, rout, pout, w, bias, oy + y0, n, xin);
     }
     {   // leftover columns: lane j's pixel row is kRows * g + ro(j); only the
         // 18-wide layer's last group has rows past the side (read clamped, not stored)
@@ -301,7 +343,12 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
         const int yc = min(y0, WO - 1);
         const uint16_t* rin[K::kBlocks];
 #pragma unroll
-        for (int tp = 0; tp < K::kBlocks; tp++) rin[tp] = cin + chunk_off(yc + tpy[tp], x + tpx[tp], WI, ch);
+        for (int b = 0; b < K::kBlocks; b++) {
+            int ty, tx;
+            bool zero;
+            S::tap(b, kb, ty, tx, zero);
+            rin[b] = cin + chunk_off(yc + ty, x + tx, WI, ch);
+        }
         uint16_t* const o = out + chunk_off(yc, x, PO, kb >> 1) + 4 * (kb & 1);
 #pragma unroll
         for (int mm = 0; mm < (M::kGroups + kFWaves - 1) / kFWaves; mm++) {
@@ -312,11 +359,12 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
 #pragma unroll
             for (int h = 0; h < K::kHalves; h++) acc[h] = bias[h];
 #pragma unroll
-            for (int tp = 0; tp < K::kBlocks; tp++) {
-                const bf16x8 v = as_bf16x8(*(const uint4*)(rin[tp] + mm * kStep * WI * 16));
+            for (int i = 0; i < K::kBlocks; i++) {
+                const int b = S::order(i);
+                const bf16x8 v = as_bf16x8(*(const uint4*)(rin[b] + mm * kStep * WI * 16));
 #pragma unroll
                 for (int h = 0; h < K::kHalves; h++)
-                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][tp], v, acc[h], 0, 0, 0);
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][b], v, acc[h], 0, 0, 0);
             }
             const int iy = oy + y, ix = ox + x;
             const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
@@ -480,37 +528,51 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
     // conv C -> 1 + bias + residual on the matrix cores: the K blocks and
     // operand reads of the mid layers over the a3 region, with W4 in every
     // row of A, so every row of D is the output; row 0 (lanes 0..15,
-    // register 0) is used.  Wave w takes output rows w, w + 4, w + 8, w + 12.
+    // register 0) is used.  Wave w sweeps output rows 4w .. 4w + 3.
     float dsum = 0.0f;  // training: this thread's squared errors
     {
+        using S = Sweep<C>;
+        constexpr int R = 4;
         const int ch = kb % K::kChunks;
+        const int y0 = R * wave;
         bf16x8 w4a[K::kBlocks];
-        const uint16_t* rin[K::kBlocks];
 #pragma unroll
-        for (int tp = 0; tp < K::kBlocks; tp++) {
-            const int tap = K::kTapsPer * tp + kb / K::kChunks, tc = min(tap, 8);
-            const uint4 wv = pair_swap(*(const uint4*)(w4 + tc * (C / 2) + 4 * ch));  // W4[tap][8ch .. 8ch+7]
-            w4a[tp] = as_bf16x8(tap < 9 ? wv : make_uint4(0, 0, 0, 0));
-            rin[tp] = sa + (ch >> 1) * kPlA + chunk_off(wave + tc / 3, j16 + tc % 3, kP3, ch & 1);
+        for (int q = 0; q < K::kBlocks; q++) {
+            int ty, tx;
+            bool zero;
+            S::tap(q, kb, ty, tx, zero);
+            const uint4 wv = pair_swap(*(const uint4*)(w4 + (3 * ty + tx) * (C / 2) + 4 * ch));  // W4[tap][8ch .. 8ch+7]
+            w4a[q] = as_bf16x8(zero ? make_uint4(0, 0, 0, 0) : wv);
         }
+        const uint16_t* kin[S::kKinds];
+#pragma unroll
+        for (int k = 0; k < S::kKinds; k++)
+            kin[k] = sa + (ch >> 1) * kPlA + chunk_off(y0 + S::krow(k, kb), j16 + S::kcol(k, kb), kP3, ch & 1);
         const float b4 = prm[L.B4];
-        f32x4 acc[4];  // rows wave + 4r
+        f32x4 acc[R];
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[r] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+        for (int y = 0; y < R; y++) acc[y] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int tp = 0; tp < K::kBlocks; tp++) {
-            bf16x8 v[4];
+        for (int r = 0; r < R + 2; r++) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) v[r] = as_bf16x8(*(const uint4*)(rin[tp] + 4 * r * kP3 * 16));
+            for (int k = 0; k < S::kKinds; k++) {
+                bool need = false;
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4a[tp], v[r], acc[r], 0, 0, 0);
-        }
-        if (kb == 0) {
+                for (int q = 0; q < K::kBlocks; q++)
+                    if (S::kind(q) == k && r - S::dy(q) >= 0 && r - S::dy(q) < R) need = true;
+                if (!need) continue;
+                const bf16x8 v = as_bf16x8(*(const uint4*)(kin[k] + r * kP3 * 16));
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int y = wave + 4 * r;
+                for (int q = 0; q < K::kBlocks; q++) {
+                    const int y = r - S::dy(q);
+                    if (S::kind(q) != k || y < 0 || y >= R) continue;
+                    acc[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w4a[q], v, acc[y], 0, 0, 0);
+                }
+            }
+            if (r >= 2 && kb == 0) {
+                const int y = y0 + r - 2;
                 const size_t p = ((size_t)b * n + i0 + y) * n + j0 + j16;
-                const float o = img[(size_t)(i0 + y) * n + j0 + j16] + (acc[r][0] + b4);
+                const float o = img[(size_t)(i0 + y) * n + j0 + j16] + (acc[r - 2][0] + b4);
                 out[p] = o;
                 if (kTrain && target) {
                     const float d = o - target[p];
