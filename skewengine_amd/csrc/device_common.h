@@ -12,6 +12,11 @@
 
 namespace ske {
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (gfx950 included):
+// 32-bit data format, no swizzle.  With stride 0, a load whose byte offset
+// is at or past num_records returns zero instead of touching memory.
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
 // Value of skewed cell (r, j): the two-tap scatter of
 // src/skewEngine.cuh:612-621 restated as a gather.  Source rows are visited
 // in ascending order there, so the w tap of row r-t-1 lands before the

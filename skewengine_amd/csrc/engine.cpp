@@ -65,6 +65,7 @@ struct DevPlanBuf {
         v.weight = weight.p;
         v.fl = fl.p;
         v.maxL = ip.maxL;
+        v.maxH = ip.maxH;
         return v;
     }
 };

@@ -70,18 +70,11 @@ int ske_fbp_create(int n, int n_angles, int n_det, const float* angles, int max_
         cs[a] = make_float2((float)cos(th), (float)sin(th));
     }
     SKE_TRY(f->cs.upload(cs.data(), cs.size(), f->stream));
-    std::vector<int> ang;
-    for (int a = 0; a < n_angles; a++)
-        if (fabsf(cs[a].x) >= fabsf(cs[a].y)) ang.push_back(a);
-    f->n_bins = (int)ang.size();
-    for (int a = 0; a < n_angles; a++)
-        if (!(fabsf(cs[a].x) >= fabsf(cs[a].y))) ang.push_back(a);
-    SKE_TRY(f->ang.upload(ang.data(), ang.size(), f->stream));
     ramlak_taps(n_det, 2.0 / n, f->taps_host);
     SKE_TRY(f->taps.upload(f->taps_host.data(), f->taps_host.size(), f->stream));
     for (int w = 0; w < 4; w++) SKE_TRY(f->buf[w].ensure(f->elems(w, max_batch)));
-    const size_t np = (size_t)n + 2 * kProjPad;
-    SKE_TRY(f->padded.ensure(np * np * (size_t)max_batch));
+    f->pt = proj_tiling(n);
+    SKE_TRY(f->part.ensure((size_t)max_batch * n_angles * f->pt.ntx * f->pt.ntx * f->pt.kw));
     SKE_TRY(f->timer.sync(f->stream));
     *out = f.release();
     return SKE_OK;
@@ -122,8 +115,8 @@ int ske_fbp_buffer(ske_fbp* f, int which, float** d_ptr) {
 int ske_fbp_project(ske_fbp* f, int batch) {
     SKE_TRY(check_batch(f, batch));
     return f->timer.launch(f->stream, "fbp_project", [&] {
-        return launch_fbp_project(f->buf[SKE_FBP_IMAGE].p, f->padded.p, f->buf[SKE_FBP_SINO].p, f->cs.p, f->ang.p,
-                                  f->n_bins, f->ang.p + f->n_bins, f->g.nang - f->n_bins, f->g, batch, f->stream);
+        return launch_fbp_project(f->buf[SKE_FBP_IMAGE].p, f->part.p, f->buf[SKE_FBP_SINO].p, f->cs.p, f->g, f->pt,
+                                  batch, f->stream);
     });
 }
 

@@ -13,12 +13,11 @@ struct ske_fbp {
     ske::FbpGeom g{};
     int max_batch = 0;
     ske::DevBuf<float2> cs;
-    ske::DevBuf<int> ang;  // angle indices: those with |cos| >= |sin| first (n_bins of them), then the rest
-    int n_bins = 0;
     ske::DevBuf<float> taps;
     std::vector<float> taps_host;
     ske::DevBuf<float> buf[4];  // image, sinogram, filtered, reconstruction
-    ske::DevBuf<float> padded;  // the projector's zero-bordered image copy
+    ske::ProjTiling pt{};
+    ske::DevBuf<float> part;    // the projector's partial sums per tile
     ske::LaunchTimer timer;
     ~ske_fbp() {
         if (stream) {

@@ -7,9 +7,10 @@
 //   bins    s_k = (k - (ndet-1)/2) * delta.
 //
 //   project     p(a,k) = delta * sum_t f(s_k cos - t sin, s_k sin + t cos),
-//               t on a delta grid over the chord of the ray through the
-//               image square, f bilinear (zero outside); a wave's lanes
-//               cover 8 bins x 8 samples, a compact block of the image;
+//               t on a delta grid of nt = ndet samples, f bilinear (zero
+//               outside); the image is cut into tiles staged in LDS, each
+//               tile's share of every ray summed there, the shares added in
+//               tile order;
 //   filter      q(a,k) = sum_k' p(a,k') h(k-k'), h = delta * Ram-Lak
 //               (h0 = 1/(4 delta^2), odd m: -1/(m^2 pi^2 delta^2), even: 0) or
 //               learned taps, as a GEMM with a Toeplitz matrix on the FP32
@@ -24,56 +25,13 @@
 
 namespace ske {
 
-// The projector reads a copy of each image with a zero border of kProjPad
-// pixels, so the bilinear taps (ray_sample) need no bounds tests: the samples of
-// ray_range lie within 1 + 2 delta of the centre along each axis, i.e. their
-// taps within 3 pixels of the image (checked exhaustively on the CPU for
-// n = 2..1024, 2..2n+3 bins and arbitrary angles), and zeros there equal the
-// zero-outside rule.
-__global__ __launch_bounds__(256) void k_fbp_pad(const float* __restrict__ img, float* __restrict__ pad, int n,
-                                                 int np, size_t total) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const size_t per = (size_t)np * np;
-    const size_t b = e / per;
-    const int r = (int)((e - b * per) / np) - kProjPad, c = (int)(e % np) - kProjPad;
-    pad[e] = (r >= 0 && r < n && c >= 0 && c < n) ? img[b * n * n + (size_t)r * n + c] : 0.0f;
-}
-
-// Sample range [i0, i1] of the ray of bin k at angle (c, sn): the samples
-// t_i = (i - (nt-1)/2) delta inside the chord of the line x c + y sn = s
-// through the square [-B, B]^2, B = 1 + delta (the bilinear image is zero
-// farther than 1 + delta/2 from the centre along either axis, so samples
-// outside add exact zeros).  x = s c - t sn, y = s sn + t c.  Empty: i1 < i0.
-__device__ __forceinline__ void ray_range(const FbpGeom& g, float s, float c, float sn, int& i0, int& i1) {
-    const float B = 1.0f + g.delta;
-    float tlo = -g.tmax, thi = g.tmax;
-    if (fabsf(sn) > 1e-7f) {
-        const float t1 = (s * c - B) / sn, t2v = (s * c + B) / sn;
-        tlo = fmaxf(tlo, fminf(t1, t2v));
-        thi = fminf(thi, fmaxf(t1, t2v));
-    } else if (fabsf(s * c) > B) {
-        thi = tlo - 1.0f;
-    }
-    if (fabsf(c) > 1e-7f) {
-        const float t1 = (-B - s * sn) / c, t2v = (B - s * sn) / c;
-        tlo = fmaxf(tlo, fminf(t1, t2v));
-        thi = fminf(thi, fmaxf(t1, t2v));
-    } else if (fabsf(s * sn) > B) {
-        thi = tlo - 1.0f;
-    }
-    i0 = 0;
-    i1 = -1;
-    if (thi >= tlo) {
-        const float half = 0.5f * (float)(g.nt - 1);
-        i0 = max((int)ceilf(tlo / g.delta + half) - 1, 0);
-        i1 = min((int)floorf(thi / g.delta + half) + 1, g.nt - 1);
-    }
-}
-
-// Sample i of the ray of bin offset s at angle (c, sn) in pixel coordinates:
-// with t = (i - half) delta, u = (s c - t sn + 1)/delta - 1/2 = U0 - i sn and
+// Sample i of the ray of bin offset s at angle (c, sn) in pixel coordinates
+// (u: column, v: row): with t = (i - half) delta,
+// u = (s c - t sn + 1)/delta - 1/2 = U0 - i sn and
 // v = (1 - s sn - t c)/delta - 1/2 = V0 - i c, so a sample costs two fma.
+// Its bilinear base pixel is (floor v, floor u); it reads (v, u) .. (v+1,
+// u+1), zero outside the image.  In exact arithmetic, with P = k - det_c,
+// Q = half - i and m = n/2 - 1/2: u - m = P c + Q sn, v - m = -P sn + Q c.
 struct RayCoord {
     float u0, v0;
 };
@@ -83,78 +41,169 @@ __device__ __forceinline__ RayCoord ray_coord(const FbpGeom& g, float s, float c
     return RayCoord{((s * c + 1.0f) * inv - 0.5f) + half * sn, ((1.0f - s * sn) * inv - 0.5f) + half * c};
 }
 
-__device__ __forceinline__ float ray_sample(const float* __restrict__ pim, int np, const RayCoord& rc, float c,
-                                            float sn, int i) {
-    const float fi = (float)i;
-    const float u = __builtin_fmaf(-fi, sn, rc.u0), v = __builtin_fmaf(-fi, c, rc.v0);
-    const float fu = floorf(u), fv = floorf(v);
-    // 24-bit multiply: row and np are below 2^14, the offset below 2^31
-    const float* r = pim + (__mul24((int)fv + kProjPad, np) + ((int)fu + kProjPad));
-    const float wu = u - fu, wv = v - fv;
-    const float a = r[0], b = r[1], cc = r[np], d = r[np + 1];
-    const float top = __builtin_fmaf(wu, b - a, a);
-    const float bot = __builtin_fmaf(wu, d - cc, cc);
-    return __builtin_fmaf(wv, bot - top, top);
+// The projector in LDS tiles.  A sample contributes only if one of its taps
+// is an image pixel, i.e. its base pixel lies in [-1, n)^2; that square is
+// cut into ntx x ntx tiles of side ts (ProjTiling).  A workgroup stages one
+// tile of one slice -- base pixels and their right/lower neighbours, (ts+1)^2
+// values, zero outside the image -- in LDS once, then for each angle of its
+// chunk sums, per bin, the samples whose base pixel lies in the tile: every
+// sample is counted in exactly one tile, with the value the direct sampling
+// gives (same coordinates, same taps, same lerp).  The partial sums go to
+// part[slice][angle][tile][k - klo]; k_fbp_project_sum adds each ray's
+// partials in tile order.  Bins and samples a tile can hold are bounded with
+// margins (tile_krange, sample_range) and every sample is tested exactly.
+constexpr int kPtPitch = 136;  // LDS row pitch (floats) of a staged tile, >= kPtMax + 1
+constexpr int kPtWaves = 8;
+
+// Bins [klo, khi] whose rays can sample a base pixel of [x0, x1) x [y0, y1):
+// P = (u - m) c - (v - m) sn over the corners, two bins of margin.
+__device__ __forceinline__ void tile_krange(const FbpGeom& g, float c, float sn, int x0, int x1, int y0, int y1,
+                                            int& klo, int& khi) {
+    const float m = 0.5f * (float)g.n - 0.5f;
+    const float ua = (float)x0 - m, ub = (float)x1 - m, va = (float)y0 - m, vb = (float)y1 - m;
+    const float p0 = ua * c - va * sn, p1 = ub * c - va * sn, p2 = ua * c - vb * sn, p3 = ub * c - vb * sn;
+    const float pmin = fminf(fminf(p0, p1), fminf(p2, p3)), pmax = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+    klo = max((int)floorf(pmin + g.det_c) - 2, 0);
+    khi = min((int)ceilf(pmax + g.det_c) + 2, g.ndet - 1);
 }
 
-// One wave per 8 bins of one angle and slice: lane (kb, ib) = (lane % 8,
-// lane / 8) takes the samples i = i_lo + ib, i_lo + ib + 8, ... of bin
-// k0 + kb, where [i_lo, i_hi] covers the 8 bins' sample ranges; samples
-// outside the lane's own range [i0, i1] are skipped (they would add exact
-// zeros, and may lie beyond the zero border).  At every pass the 64 lanes'
-// samples form an 8 x 8 block of (bin, sample) points, a rotated square of
-// side 8 delta: their bilinear taps fall on a handful of image rows and
-// cache lines at every angle.  Four passes' gathers are in flight together;
-// each lane sums its samples in order, then the 8 lanes of a bin are added
-// in a fixed butterfly order.
-__global__ __launch_bounds__(256) void k_fbp_project(const float* __restrict__ pimg, float* __restrict__ sino,
-                                                     const float2* __restrict__ cs, const int* __restrict__ ang,
-                                                     FbpGeom g) {
-    const int lane = threadIdx.x & 63;
-    const int kb = lane & 7, ib = lane >> 3;
-    const int kbase = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 8;
-    const int a = ang[blockIdx.y];
-    const int b = blockIdx.z;
-    if (kbase >= g.ndet) return;  // uniform per wave
-    const int k = min(kbase + kb, g.ndet - 1);
-    const float2 t2 = cs[a];
-    const float s = ((float)k - g.det_c) * g.delta;
-    int i0, i1;
-    ray_range(g, s, t2.x, t2.y, i0, i1);
-    if (kbase + kb >= g.ndet) i1 = i0 - 1;  // padding lanes: no samples
-    const RayCoord rc = ray_coord(g, s, t2.x, t2.y);
-    // common range of the wave's 8 bins (every lane holds its own bin's range)
-    int lo = i1 >= i0 ? i0 : INT_MAX, hi = i1 >= i0 ? i1 : INT_MIN;
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
-        lo = min(lo, __shfl_xor(lo, off, 64));
-        hi = max(hi, __shfl_xor(hi, off, 64));
+// Samples [ilo, ihi] of bin offset P whose base pixel can lie in the tile:
+// the line's chord through the tile's square widened by one pixel (Q from
+// both slabs), two samples of margin; empty: ihi < ilo.
+__device__ __forceinline__ void sample_range(const FbpGeom& g, float c, float sn, float P, int x0, int x1, int y0,
+                                             int y1, int& ilo, int& ihi) {
+    const float m = 0.5f * (float)g.n - 0.5f;
+    const float X0 = (float)x0 - m - 1.0f, X1 = (float)x1 - m + 1.0f;
+    const float Y0 = (float)y0 - m - 1.0f, Y1 = (float)y1 - m + 1.0f;
+    float qlo = -1e30f, qhi = 1e30f;
+    const float pc = P * c, ps = P * sn;
+    if (fabsf(sn) > 1e-6f) {  // u - m = P c + Q sn in [X0, X1]
+        const float q1 = (X0 - pc) / sn, q2 = (X1 - pc) / sn;
+        qlo = fmaxf(qlo, fminf(q1, q2));
+        qhi = fminf(qhi, fmaxf(q1, q2));
+    } else if (pc < X0 || pc > X1) {
+        qhi = qlo - 1.0f;
     }
-    const int np = g.n + 2 * kProjPad;
-    const float* im = pimg + (size_t)b * np * np;
-    float acc = 0.0f;
-    if (lo <= hi) {
-        int i = lo + ib;
-        for (; i + 24 <= hi; i += 32) {
-            float v[4];
+    if (fabsf(c) > 1e-6f) {  // v - m = -P sn + Q c in [Y0, Y1]
+        const float q1 = (Y0 + ps) / c, q2 = (Y1 + ps) / c;
+        qlo = fmaxf(qlo, fminf(q1, q2));
+        qhi = fminf(qhi, fmaxf(q1, q2));
+    } else if (-ps < Y0 || -ps > Y1) {
+        qhi = qlo - 1.0f;
+    }
+    ilo = 0;
+    ihi = -1;
+    if (qhi >= qlo) {
+        const float half = 0.5f * (float)(g.nt - 1);
+        ilo = max((int)floorf(half - qhi) - 2, 0);
+        ihi = min((int)ceilf(half - qlo) + 2, g.nt - 1);
+    }
+}
+
+// One workgroup per (tile, chunk of kPtAngles angles, slice).  Work items are
+// (angle, block of 8 bins) of the chunk, taken by the waves in turn; lane
+// (kb, ib) = (lane % 8, lane / 8) sums the samples ilo + ib, ilo + ib + 8, ...
+// of bin k0 + kb over the union of the 8 bins' sample ranges (an 8 x 8 block
+// of (bin, sample) points a pass: a compact rotated square of base pixels),
+// four passes in flight; then the 8 lanes of a bin are added in a fixed
+// butterfly order.
+__global__ __launch_bounds__(64 * kPtWaves) void k_fbp_project_tiles(const float* __restrict__ img,
+                                                                      float* __restrict__ part,
+                                                                      const float2* __restrict__ cs, FbpGeom g,
+                                                                      ProjTiling pt) {
+    __shared__ float L[(kPtMax + 1) * kPtPitch];
+    const int T = blockIdx.x, b = blockIdx.z;
+    const int x0 = -1 + (T % pt.ntx) * pt.ts, y0 = -1 + (T / pt.ntx) * pt.ts;
+    const int x1 = min(x0 + pt.ts, g.n), y1 = min(y0 + pt.ts, g.n);  // base pixels [x0, x1) x [y0, y1)
+    const float* im = img + (size_t)b * g.n * g.n;
+    const int side = pt.ts + 1;
+    for (int e = threadIdx.x; e < side * side; e += blockDim.x) {
+        const int r = e / side, q = e - r * side;
+        const int y = y0 + r, x = x0 + q;
+        L[r * kPtPitch + q] = (y >= 0 && y < g.n && x >= 0 && x < g.n) ? im[(size_t)y * g.n + x] : 0.0f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, kb = lane & 7, ib = lane >> 3;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int a0 = blockIdx.y * kPtAngles, na = min(kPtAngles, g.nang - a0);
+    const int w = x1 - x0, h = y1 - y0;
+    int item = wave;  // flat index over (angle, bin block) pairs of the chunk
+    int base = 0;     // items of the angles before aa
+    for (int aa = 0; aa < na; ++aa) {
+        const int a = a0 + aa;
+        const float2 t2 = cs[a];
+        const float c = t2.x, sn = t2.y;
+        int klo, khi;
+        tile_krange(g, c, sn, x0, x1, y0, y1, klo, khi);
+        const int nkb = khi >= klo ? (khi - klo + 8) / 8 : 0;
+        float* const out = part + (((size_t)b * g.nang + a) * pt.ntx * pt.ntx + T) * pt.kw;
+        for (; item < base + nkb; item += kPtWaves) {
+            const int k = klo + 8 * (item - base) + kb;
+            const bool kin = k <= khi;
+            const float P = (float)k - g.det_c;
+            int i0, i1;
+            sample_range(g, c, sn, P, x0, x1, y0, y1, i0, i1);
+            if (!kin) i1 = i0 - 1;
+            int lo = i1 >= i0 ? i0 : INT_MAX, hi = i1 >= i0 ? i1 : INT_MIN;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int iu = i + 8 * u;
-                v[u] = 0.0f;
-                if (iu >= i0 && iu <= i1) v[u] = ray_sample(im, np, rc, t2.x, t2.y, iu);  // lanes masked: no load
+            for (int off = 1; off < 8; off <<= 1) {
+                lo = min(lo, __shfl_xor(lo, off, 64));
+                hi = max(hi, __shfl_xor(hi, off, 64));
             }
-            acc += v[0];
-            acc += v[1];
-            acc += v[2];
-            acc += v[3];
+            const RayCoord rc = ray_coord(g, P * g.delta, c, sn);
+            auto sample = [&](int i) -> float {
+                const float fi = (float)i;
+                const float u = __builtin_fmaf(-fi, sn, rc.u0), v = __builtin_fmaf(-fi, c, rc.v0);
+                const float fu = floorf(u), fv = floorf(v);
+                const int iu = (int)fu - x0, iv = (int)fv - y0;
+                if (!kin || (unsigned)iu >= (unsigned)w || (unsigned)iv >= (unsigned)h) return 0.0f;
+                const float* r = L + iv * kPtPitch + iu;
+                const float wu = u - fu, wv = v - fv;
+                const float ta = r[0], tb = r[1], tc = r[kPtPitch], td = r[kPtPitch + 1];
+                const float top = __builtin_fmaf(wu, tb - ta, ta);
+                const float bot = __builtin_fmaf(wu, td - tc, tc);
+                return __builtin_fmaf(wv, bot - top, top);
+            };
+            float acc = 0.0f;
+            if (lo <= hi) {
+                int i = lo + ib;
+                for (; i + 24 <= hi; i += 32) {
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v[u] = sample(i + 8 * u);
+                    acc += v[0];
+                    acc += v[1];
+                    acc += v[2];
+                    acc += v[3];
+                }
+                for (; i <= hi; i += 8) acc += sample(i);
+            }
+            acc += __shfl_xor(acc, 8, 64);
+            acc += __shfl_xor(acc, 16, 64);
+            acc += __shfl_xor(acc, 32, 64);
+            if (ib == 0 && kin) out[k - klo] = acc;
         }
-        for (; i <= hi; i += 8)
-            if (i >= i0 && i <= i1) acc += ray_sample(im, np, rc, t2.x, t2.y, i);
+        base += nkb;
     }
-    acc += __shfl_xor(acc, 8, 64);
-    acc += __shfl_xor(acc, 16, 64);
-    acc += __shfl_xor(acc, 32, 64);
-    if (ib == 0 && kbase + kb < g.ndet) sino[((size_t)b * g.nang + a) * g.ndet + kbase + kb] = acc * g.delta;
+}
+
+// sino(a, k) = delta * the sum of the partials of the tiles its ray crosses,
+// in tile order.
+__global__ __launch_bounds__(256) void k_fbp_project_sum(const float* __restrict__ part, float* __restrict__ sino,
+                                                         const float2* __restrict__ cs, FbpGeom g, ProjTiling pt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y, b = blockIdx.z;
+    if (k >= g.ndet) return;
+    const float2 t2 = cs[a];
+    const int nt2 = pt.ntx * pt.ntx;
+    const float* const p = part + ((size_t)b * g.nang + a) * nt2 * pt.kw;
+    float acc = 0.0f;
+    for (int T = 0; T < nt2; ++T) {
+        const int x0 = -1 + (T % pt.ntx) * pt.ts, y0 = -1 + (T / pt.ntx) * pt.ts;
+        int klo, khi;
+        tile_krange(g, t2.x, t2.y, x0, min(x0 + pt.ts, g.n), y0, min(y0 + pt.ts, g.n), klo, khi);
+        if (k >= klo && k <= khi) acc += p[(size_t)T * pt.kw + (k - klo)];
+    }
+    sino[((size_t)b * g.nang + a) * g.ndet + k] = acc * g.delta;
 }
 
 // The filter as a GEMM on the matrix cores: Q = P T over the sinogram rows
@@ -277,16 +326,13 @@ __global__ __launch_bounds__(256) void k_fbp_backproject(const float* __restrict
     if (in) img[(size_t)b * g.n * g.n + (size_t)i * g.n + j] = acc * scale;
 }
 
-hipError_t launch_fbp_project(const float* img, float* padded, float* sino, const float2* cs, const int* ang_bins,
-                              int n_bins, const int* ang_rays, int n_rays, const FbpGeom& g, int batch, hipStream_t s) {
-    const int np = g.n + 2 * kProjPad;
-    const size_t total = (size_t)batch * np * np;
-    hipLaunchKernelGGL(k_fbp_pad, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, padded, g.n, np, total);
-    const unsigned groups = (unsigned)((g.ndet + 31) / 32);  // 4 waves x 8 bins per workgroup
-    if (n_bins > 0)
-        hipLaunchKernelGGL(k_fbp_project, dim3(groups, n_bins, batch), dim3(256), 0, s, padded, sino, cs, ang_bins, g);
-    if (n_rays > 0)
-        hipLaunchKernelGGL(k_fbp_project, dim3(groups, n_rays, batch), dim3(256), 0, s, padded, sino, cs, ang_rays, g);
+hipError_t launch_fbp_project(const float* img, float* part, float* sino, const float2* cs, const FbpGeom& g,
+                              const ProjTiling& pt, int batch, hipStream_t s) {
+    if (pt.ts < 1 || pt.ts > kPtMax || pt.ntx * pt.ts < g.n + 1) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(pt.ntx * pt.ntx), (unsigned)((g.nang + kPtAngles - 1) / kPtAngles), (unsigned)batch);
+    hipLaunchKernelGGL(k_fbp_project_tiles, grid, dim3(64 * kPtWaves), 0, s, img, part, cs, g, pt);
+    hipLaunchKernelGGL(k_fbp_project_sum, dim3((unsigned)((g.ndet + 255) / 256), (unsigned)g.nang, (unsigned)batch),
+                       dim3(256), 0, s, part, sino, cs, g, pt);
     return hipGetLastError();
 }
 

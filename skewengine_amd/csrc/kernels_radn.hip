@@ -14,6 +14,8 @@
 //            so the parallel stage computes exactly the reference's values.
 //   The smear back (kernelRADN.h:252-257) is folded into the deskew, which
 //   reads q_r for every cell of row r in [first, last).
+#include <limits.h>
+
 #include "device_common.h"
 #include "launch.h"
 
@@ -67,8 +69,116 @@ __global__ __launch_bounds__(256) void k_radn_rowsum(const SecDesc* __restrict__
     if (lane == 0) P[sd.q + w.row] = acc;
 }
 
+// The fused row sums (no materialised skewInput): one wave per 64
+// consecutive skewed rows of one sector, lane = row.  Each lane adds
+// S(r, j) over its own [first, last) in ascending j in double -- the
+// reference's own order (kernelRADN.h:227-232), so P is the reference's sum
+// bit for bit -- with S(r, j) computed as skew_cell does.  The two taps of
+// cell (r, j) lie in column j of the oriented copy, rows r - t_j - 1 and
+// r - t_j; a column of one oriented copy is a row of another (the partner,
+// read backwards for types 0 and 3):
+//   type 0: in0[a][j] = in2[j][dimy-1-a]        type 1: in1[a][j] = in3[dimy-1-j][a]
+//   type 2: in2[a][j] = in0[dimy-1-j][a]        type 3: in3[a][j] = in2[dimx-1-j][dimy-1-a]
+// so at every j the 64 lanes' taps are 64 consecutive floats of one row and
+// t_j, w_j are wave-uniform (scalar loads).  The byte offset of the lane's
+// lower-addressed tap is A + B j + C t_j + D: three sector constants and a
+// per-lane term.  The wave walks the union of its rows' ranges; cells
+// outside a lane's own range are read but add zero (acc + 0 = acc: acc is
+// never -0).
+constexpr int kColsUnroll = 8;
+template <bool kRev>
+__device__ __forceinline__ double rowsum_cols_walk(const int* __restrict__ tt, const float* __restrict__ ww,
+                                                   __amdgpu_buffer_rsrc_t src, int A, int B, int C, int D, int r,
+                                                   int dim_o, int f, int l, int lo, int hi) {
+    auto cell = [&](int j, int t, float w, uint2 pr) -> float {
+        // kRev: the tap of row r - t at the lower address
+        const float x1 = __uint_as_float(kRev ? pr.y : pr.x), x0 = __uint_as_float(kRev ? pr.x : pr.y);
+        const int i0 = r - t;
+        const float s = ((unsigned)(i0 - 1) < (unsigned)dim_o) ? 0.0f + w * x1 : 0.0f;
+        const float v = ((unsigned)i0 < (unsigned)dim_o) ? s + (1.0f - w) * x0 : s;
+        return ((unsigned)(j - f) < (unsigned)(l - f)) ? v : 0.0f;
+    };
+    auto load = [&](int j, int t) -> uint2 {
+        const int o = A + B * j + C * t + D;
+        return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(src, o, 0, 0),
+                          __builtin_amdgcn_raw_buffer_load_b32(src, o + 4, 0, 0));
+    };
+    double acc = 0.0;
+    int j = lo;
+    for (; j + kColsUnroll <= hi; j += kColsUnroll) {
+        int t[kColsUnroll];
+        float w[kColsUnroll];
+        uint2 pr[kColsUnroll];
+#pragma unroll
+        for (int u = 0; u < kColsUnroll; ++u) {
+            t[u] = tt[j + u];
+            w[u] = ww[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kColsUnroll; ++u) pr[u] = load(j + u, t[u]);
+#pragma unroll
+        for (int u = 0; u < kColsUnroll; ++u) acc += (double)cell(j + u, t[u], w[u], pr[u]);
+    }
+    for (; j < hi; ++j) {
+        const int t = tt[j];
+        acc += (double)cell(j, t, ww[j], load(j, t));
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_radn_rowsum_cols(const SecDesc* __restrict__ secs,
+                                                          const float* __restrict__ in4,
+                                                          const int* __restrict__ target,
+                                                          const float* __restrict__ weight,
+                                                          const uint32_t* __restrict__ fl, double* __restrict__ P) {
+    const SecDesc sd = secs[blockIdx.y];
+    const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (r0 >= sd.H) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int r = r0 + lane;
+    const bool live = r < sd.H;
+    int f = 0, l = 0;
+    if (live) {
+        const uint32_t v = fl[sd.rows + r];
+        f = fl_first(v);
+        l = fl_last(v);
+    }
+    if (!(live && l > f)) f = l = 0;  // no cell of this lane is added
+    int lo = l > f ? f : INT_MAX, hi = l > f ? l : INT_MIN;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
+    const int dim_o = sd.dim_o, dim_i = sd.dim_i, type = sd.type;
+    const int partner = type == 0 ? 2 : type == 1 ? 3 : type == 2 ? 0 : 2;
+    // the partner copy as a raw buffer of N floats: a tap outside it (a row
+    // index outside the image, whose cell is masked anyway) reads zero
+    const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(in4 + sd.img + (size_t)partner * sd.N), 0, sd.N * 4, kBufferRsrcWord3);
+    const int* const tt = target + sd.tab;
+    const float* const ww = weight + sd.tab;
+    double acc;
+    if (type == 0)  // partner row j, position dim_o-1-a: tap r-t at 4 (j dim_o + dim_o-1 + t - r)
+        acc = rowsum_cols_walk<true>(tt, ww, src, 4 * (dim_o - 1), 4 * dim_o, 4, -4 * r, r, dim_o, f, l, lo, hi);
+    else if (type == 3)  // partner row dim_i-1-j, position dim_o-1-a
+        acc = rowsum_cols_walk<true>(tt, ww, src, 4 * ((dim_i - 1) * dim_o + dim_o - 1), -4 * dim_o, 4, -4 * r, r,
+                                     dim_o, f, l, lo, hi);
+    else  // types 1, 2: partner row dim_i-1-j, position a: tap r-t-1 at 4 ((dim_i-1-j) dim_o + r - t - 1)
+        acc = rowsum_cols_walk<false>(tt, ww, src, 4 * ((dim_i - 1) * dim_o - 1), -4 * dim_o, -4, 4 * r, r, dim_o,
+                                      f, l, lo, hi);
+    if (live) P[sd.q + r] = acc;
+}
+
 hipError_t launch_radn_rowsum(const DevPlan& p, const float* in4, const float* Sbuf, double* P, hipStream_t s) {
     if (p.nwork == 0) return hipSuccess;
+    if (!Sbuf && p.nsec > 0 && p.maxH > 0) {
+        hipLaunchKernelGGL(k_radn_rowsum_cols, dim3((p.maxH + 255) / 256, p.nsec), dim3(256), 0, s, p.secs, in4,
+                           p.target, p.weight, p.fl, P);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_radn_rowsum, dim3((p.nwork + 3) / 4), dim3(256), 0, s, p.secs, p.work, p.nwork, in4,
                        p.target, p.weight, p.fl, Sbuf, P);
     return hipGetLastError();

@@ -17,6 +17,7 @@ struct DevPlan {
     const float* weight = nullptr;
     const uint32_t* fl = nullptr;  // first | last << 16 per row
     int maxL = 0;                   // longest valid row (LDS sizing)
+    int maxH = 0;                   // tallest skewed block (rows per sector)
 };
 
 enum DeskewSource { kDeskewDense = 0, kDeskewArena = 1, kDeskewRadon = 2 };

@@ -258,6 +258,7 @@ int execute_noalloc(int kernel_id, int n, const float* d_input, float* d_output,
         v.weight = reinterpret_cast<const float*>(base + ch.weight);
         v.fl = reinterpret_cast<const uint32_t*>(base + ch.fl);
         v.maxL = ch.ip.maxL;
+        v.maxH = ch.ip.maxH;
         switch (kernel_id) {
         case SKE_KERNEL_SDEM: {
             // the engine's default launch shape (two waves for short rows)
