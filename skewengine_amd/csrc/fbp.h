@@ -20,7 +20,7 @@ struct FbpGeom {
 // of side ts <= kPtMax; kw partial sums per (slice, angle, tile), enough for
 // every bin that can cross a tile (ts sqrt(2) plus margins).
 constexpr int kPtMax = 129;
-constexpr int kPtAngles = 6;  // angles per projector workgroup
+constexpr int kPtAngles = 12;  // angles per projector workgroup
 struct ProjTiling {
     int ntx;
     int ts;

@@ -35,9 +35,9 @@ namespace ske {
 struct RayCoord {
     float u0, v0;
 };
-__device__ __forceinline__ RayCoord ray_coord(const FbpGeom& g, float s, float c, float sn) {
+// inv = 1 / delta (IEEE division)
+__device__ __forceinline__ RayCoord ray_coord(const FbpGeom& g, float s, float c, float sn, float inv) {
     const float half = 0.5f * (float)(g.nt - 1);
-    const float inv = 1.0f / g.delta;
     return RayCoord{((s * c + 1.0f) * inv - 0.5f) + half * sn, ((1.0f - s * sn) * inv - 0.5f) + half * c};
 }
 
@@ -54,6 +54,8 @@ __device__ __forceinline__ RayCoord ray_coord(const FbpGeom& g, float s, float c
 // margins (tile_krange, sample_range) and every sample is tested exactly.
 constexpr int kPtPitch = 136;  // LDS row pitch (floats) of a staged tile, >= kPtMax + 1
 constexpr int kPtWaves = 8;
+constexpr int kPtKB = 16;               // bins per work item (lanes of a pass along the bins)
+constexpr int kPtIB = 64 / kPtKB;      // samples per pass along the ray
 
 // Bins [klo, khi] whose rays can sample a base pixel of [x0, x1) x [y0, y1):
 // P = (u - m) c - (v - m) sn over the corners, two bins of margin.
@@ -69,23 +71,24 @@ __device__ __forceinline__ void tile_krange(const FbpGeom& g, float c, float sn,
 
 // Samples [ilo, ihi] of bin offset P whose base pixel can lie in the tile:
 // the line's chord through the tile's square widened by one pixel (Q from
-// both slabs), two samples of margin; empty: ihi < ilo.
-__device__ __forceinline__ void sample_range(const FbpGeom& g, float c, float sn, float P, int x0, int x1, int y0,
-                                             int y1, int& ilo, int& ihi) {
+// both slabs, rs = 1/sn and rc = 1/c rounded), two samples of margin; empty:
+// ihi < ilo.
+__device__ __forceinline__ void sample_range(const FbpGeom& g, float c, float sn, float rc, float rs, float P, int x0,
+                                             int x1, int y0, int y1, int& ilo, int& ihi) {
     const float m = 0.5f * (float)g.n - 0.5f;
     const float X0 = (float)x0 - m - 1.0f, X1 = (float)x1 - m + 1.0f;
     const float Y0 = (float)y0 - m - 1.0f, Y1 = (float)y1 - m + 1.0f;
     float qlo = -1e30f, qhi = 1e30f;
     const float pc = P * c, ps = P * sn;
     if (fabsf(sn) > 1e-6f) {  // u - m = P c + Q sn in [X0, X1]
-        const float q1 = (X0 - pc) / sn, q2 = (X1 - pc) / sn;
+        const float q1 = (X0 - pc) * rs, q2 = (X1 - pc) * rs;
         qlo = fmaxf(qlo, fminf(q1, q2));
         qhi = fminf(qhi, fmaxf(q1, q2));
     } else if (pc < X0 || pc > X1) {
         qhi = qlo - 1.0f;
     }
     if (fabsf(c) > 1e-6f) {  // v - m = -P sn + Q c in [Y0, Y1]
-        const float q1 = (Y0 + ps) / c, q2 = (Y1 + ps) / c;
+        const float q1 = (Y0 + ps) * rc, q2 = (Y1 + ps) * rc;
         qlo = fmaxf(qlo, fminf(q1, q2));
         qhi = fminf(qhi, fmaxf(q1, q2));
     } else if (-ps < Y0 || -ps > Y1) {
@@ -101,12 +104,12 @@ __device__ __forceinline__ void sample_range(const FbpGeom& g, float c, float sn
 }
 
 // One workgroup per (tile, chunk of kPtAngles angles, slice).  Work items are
-// (angle, block of 8 bins) of the chunk, taken by the waves in turn; lane
-// (kb, ib) = (lane % 8, lane / 8) sums the samples ilo + ib, ilo + ib + 8, ...
-// of bin k0 + kb over the union of the 8 bins' sample ranges (an 8 x 8 block
-// of (bin, sample) points a pass: a compact rotated square of base pixels),
-// four passes in flight; then the 8 lanes of a bin are added in a fixed
-// butterfly order.
+// (angle, block of kPtKB bins) of the chunk, taken by the waves in turn;
+// lane (kb, ib) = (lane % kPtKB, lane / kPtKB) sums the samples ilo + ib,
+// ilo + ib + kPtIB, ... of bin k0 + kb over the union of the block's sample
+// ranges (a kPtKB x kPtIB block of (bin, sample) points a pass: a compact
+// rotated patch of base pixels), four passes in flight; then the kPtIB lanes
+// of a bin are added in a fixed butterfly order.
 __global__ __launch_bounds__(64 * kPtWaves) void k_fbp_project_tiles(const float* __restrict__ img,
                                                                       float* __restrict__ part,
                                                                       const float2* __restrict__ cs, FbpGeom g,
@@ -123,64 +126,71 @@ __global__ __launch_bounds__(64 * kPtWaves) void k_fbp_project_tiles(const float
         L[r * kPtPitch + q] = (y >= 0 && y < g.n && x >= 0 && x < g.n) ? im[(size_t)y * g.n + x] : 0.0f;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, kb = lane & 7, ib = lane >> 3;
+    const int lane = threadIdx.x & 63, kb = lane % kPtKB, ib = lane / kPtKB;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = blockIdx.y * kPtAngles, na = min(kPtAngles, g.nang - a0);
-    const int w = x1 - x0, h = y1 - y0;
+    const float inv = 1.0f / g.delta;
+    // base pixels of the tile as floats, for clamping and the membership test
+    const float fx0 = (float)x0, fx1 = (float)(x1 - 1), fy0 = (float)y0, fy1 = (float)(y1 - 1);
+    const int lt = -(y0 * kPtPitch + x0);  // L[lt + y * kPtPitch + x]: image pixel (y, x) of the tile
     int item = wave;  // flat index over (angle, bin block) pairs of the chunk
     int base = 0;     // items of the angles before aa
     for (int aa = 0; aa < na; ++aa) {
         const int a = a0 + aa;
         const float2 t2 = cs[a];
         const float c = t2.x, sn = t2.y;
+        const float rc = 1.0f / c, rs = 1.0f / sn;  // used only where |c|, |sn| > 1e-6
         int klo, khi;
         tile_krange(g, c, sn, x0, x1, y0, y1, klo, khi);
-        const int nkb = khi >= klo ? (khi - klo + 8) / 8 : 0;
+        const int nkb = khi >= klo ? (khi - klo + kPtKB) / kPtKB : 0;
         float* const out = part + (((size_t)b * g.nang + a) * pt.ntx * pt.ntx + T) * pt.kw;
         for (; item < base + nkb; item += kPtWaves) {
-            const int k = klo + 8 * (item - base) + kb;
+            const int k = klo + kPtKB * (item - base) + kb;
             const bool kin = k <= khi;
             const float P = (float)k - g.det_c;
             int i0, i1;
-            sample_range(g, c, sn, P, x0, x1, y0, y1, i0, i1);
+            sample_range(g, c, sn, rc, rs, P, x0, x1, y0, y1, i0, i1);
             if (!kin) i1 = i0 - 1;
             int lo = i1 >= i0 ? i0 : INT_MAX, hi = i1 >= i0 ? i1 : INT_MIN;
 #pragma unroll
-            for (int off = 1; off < 8; off <<= 1) {
+            for (int off = 1; off < kPtKB; off <<= 1) {
                 lo = min(lo, __shfl_xor(lo, off, 64));
                 hi = max(hi, __shfl_xor(hi, off, 64));
             }
-            const RayCoord rc = ray_coord(g, P * g.delta, c, sn);
-            auto sample = [&](int i) -> float {
-                const float fi = (float)i;
-                const float u = __builtin_fmaf(-fi, sn, rc.u0), v = __builtin_fmaf(-fi, c, rc.v0);
+            const RayCoord ro = ray_coord(g, P * g.delta, c, sn, inv);
+            // branch-free, so that the reads of several samples are in flight
+            // together: a sample outside the tile reads a clamped in-tile
+            // address and is dropped
+            auto sample = [&](float fi) -> float {
+                const float u = __builtin_fmaf(-fi, sn, ro.u0), v = __builtin_fmaf(-fi, c, ro.v0);
                 const float fu = floorf(u), fv = floorf(v);
-                const int iu = (int)fu - x0, iv = (int)fv - y0;
-                if (!kin || (unsigned)iu >= (unsigned)w || (unsigned)iv >= (unsigned)h) return 0.0f;
-                const float* r = L + iv * kPtPitch + iu;
+                const float cu = __builtin_amdgcn_fmed3f(fu, fx0, fx1), cv = __builtin_amdgcn_fmed3f(fv, fy0, fy1);
+                const bool in = kin && cu == fu && cv == fv;
+                const float* r = L + (lt + __mul24((int)cv, kPtPitch) + (int)cu);
                 const float wu = u - fu, wv = v - fv;
                 const float ta = r[0], tb = r[1], tc = r[kPtPitch], td = r[kPtPitch + 1];
                 const float top = __builtin_fmaf(wu, tb - ta, ta);
                 const float bot = __builtin_fmaf(wu, td - tc, tc);
-                return __builtin_fmaf(wv, bot - top, top);
+                const float val = __builtin_fmaf(wv, bot - top, top);
+                return in ? val : 0.0f;
             };
             float acc = 0.0f;
             if (lo <= hi) {
                 int i = lo + ib;
-                for (; i + 24 <= hi; i += 32) {
+                float fi = (float)i;  // i, exactly (below 2^24)
+                for (; i + 3 * kPtIB <= hi; i += 4 * kPtIB, fi += (float)(4 * kPtIB)) {
                     float v[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) v[u] = sample(i + 8 * u);
+                    for (int u = 0; u < 4; ++u) v[u] = sample(fi + (float)(kPtIB * u));
                     acc += v[0];
                     acc += v[1];
                     acc += v[2];
                     acc += v[3];
                 }
-                for (; i <= hi; i += 8) acc += sample(i);
+                for (; i <= hi; i += kPtIB, fi += (float)kPtIB) acc += sample(fi);
             }
-            acc += __shfl_xor(acc, 8, 64);
-            acc += __shfl_xor(acc, 16, 64);
-            acc += __shfl_xor(acc, 32, 64);
+#pragma unroll
+            for (int off = kPtKB; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
             if (ib == 0 && kin) out[k - klo] = acc;
         }
         base += nkb;

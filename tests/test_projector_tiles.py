@@ -46,15 +46,16 @@ def _sample_range(n, nt, c, sn, P, x0, x1, y0, y1):
     qlo = np.full(P.shape, f32(-1e30), np.float32)
     qhi = np.full(P.shape, f32(1e30), np.float32)
     pc, ps = P * c, P * sn
-    with np.errstate(divide="ignore", invalid="ignore"):
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        rc, rs = f32(1) / c, f32(1) / sn  # reciprocals rounded to float
         if abs(sn) > f32(1e-6):
-            q1, q2 = (X0 - pc) / sn, (X1 - pc) / sn
+            q1, q2 = (X0 - pc) * rs, (X1 - pc) * rs
             qlo, qhi = np.maximum(qlo, np.minimum(q1, q2)), np.minimum(qhi, np.maximum(q1, q2))
         else:
             bad = (pc < X0) | (pc > X1)
             qhi = np.where(bad, qlo - f32(1), qhi)
         if abs(c) > f32(1e-6):
-            q1, q2 = (Y0 + ps) / c, (Y1 + ps) / c
+            q1, q2 = (Y0 + ps) * rc, (Y1 + ps) * rc
             qlo, qhi = np.maximum(qlo, np.minimum(q1, q2)), np.minimum(qhi, np.maximum(q1, q2))
         else:
             bad = (-ps < Y0) | (-ps > Y1)
@@ -119,3 +120,4 @@ def test_projector_tiling_covers_the_base_pixels():
     for n in list(range(1, 600)) + [1023, 1024, 2048]:
         ntx, ts, kw = proj_tiling(n)
         assert 1 <= ts <= KPT_MAX and ntx * ts >= n + 1 and (ntx - 1) * ts < n + 1
+
