@@ -85,6 +85,8 @@ struct Chunk {
         int range[8];
     };
     std::vector<Part> parts;
+    DevBuf<DeskewPart> dparts;  // the parts for the one-launch deskew
+    int max_n = 0;              // largest grid side among the parts
     DevBuf<SampleReq> sreq;  // sample points of the local Radon transform in this chunk
     int nreq = 0;
     int sample_version = -1;  // Core::sample_version the requests were built for
@@ -352,6 +354,15 @@ struct Core {
             if (cur->parts.empty()) return SKE_OK;
             plan_finish(cur->plan.ip, order);
             SKE_TRY(cur->plan.upload(stream));
+            std::vector<DeskewPart> dp(cur->parts.size());
+            for (size_t k = 0; k < cur->parts.size(); k++) {
+                const Image& im = imgs[cur->parts[k].img];
+                for (int r = 0; r < 8; r++) dp[k].range[r] = cur->parts[k].range[r];
+                dp[k].acc_off = im.acc_off;
+                dp[k].dimx = dp[k].dimy = im.n;
+                cur->max_n = std::max(cur->max_n, im.n);
+            }
+            SKE_TRY(cur->dparts.upload(dp.data(), dp.size(), stream));
             rp->max_so = std::max(rp->max_so, cur->plan.ip.so_total);
             rp->max_q = std::max(rp->max_q, cur->plan.ip.q_total);
             rp->maxH = std::max(rp->maxH, cur->plan.ip.maxH);
@@ -497,7 +508,6 @@ struct Core {
                     }
                     break;
                 }
-                SKE_TRY(launch("fill", [&] { return launch_fill_f64(P.p, (size_t)ip.q_total, stream); }));
                 SKE_TRY(launch("radn_rowsum", [&] { return launch_radn_rowsum(v, in4.p, nullptr, P.p, stream); }));
                 SKE_TRY(launch("radn_filter", [&] {
                     return launch_radn_filter(v, P.p, q.p, tw.p, tw_off.p, 1 << next_pow2_log(ip.maxH), stream);
@@ -509,12 +519,10 @@ struct Core {
             }
             const float* src = radon ? q.p : arena.p;
             const int source = radon ? kDeskewRadon : kDeskewArena;
-            for (const Chunk::Part& part : cp->parts) {
-                const Image& im = imgs[part.img];
-                SKE_TRY(launch("deskew", [&] {
-                    return launch_deskew(v, part.range, src, source, is_max, acc.p, im.n, im.n, im.acc_off, stream);
-                }));
-            }
+            SKE_TRY(launch("deskew", [&] {
+                return launch_deskew_parts(v, cp->dparts.p, (int)cp->parts.size(), cp->max_n, src, source, is_max,
+                                           acc.p, stream);
+            }));
         }
         const size_t out_n = out.n;
         SKE_TRY(launch("fill", [&] { return launch_fill_f32(out.p, out_n, 0.0f, stream); }));

@@ -203,15 +203,13 @@ __device__ void deskew_cell(const SecDesc* __restrict__ secs, const int* __restr
 // kRows consecutive output rows per thread: the kRows + 1 taps of a sector
 // are shared by neighbouring rows.  kU sectors per pass: their table reads,
 // then their gathers are issued together, and each row accumulates in
-// ascending sector order afterwards.
+// ascending sector order afterwards.  Orientation t of one image: sectors
+// [sb, se), accumulators acc (four dimx*dimy buffers).
 template <int kSource, int kMax, int kRows, int kU>
-__global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs, const int* __restrict__ target,
-                                                const float* __restrict__ weight, const uint32_t* __restrict__ fl,
-                                                const float* __restrict__ src, int4 rlo, int4 rhi,
-                                                float* __restrict__ acc, int dimx, int dimy) {
-    const int t = blockIdx.z;
-    const int sb = (t == 0) ? rlo.x : (t == 1) ? rlo.z : (t == 2) ? rhi.x : rhi.z;
-    const int se = (t == 0) ? rlo.y : (t == 1) ? rlo.w : (t == 2) ? rhi.y : rhi.w;
+__device__ __forceinline__ void deskew_body(const SecDesc* __restrict__ secs, const int* __restrict__ target,
+                                            const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                                            const float* __restrict__ src, int t, int sb, int se,
+                                            float* __restrict__ acc, int dimx, int dimy) {
     if (sb >= se) return;
     const bool tr = (t == 1 || t == 2);
     const int dim_i = tr ? dimy : dimx, dim_o = tr ? dimx : dimy;
@@ -275,11 +273,45 @@ __global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs
     for (int k = 0; k < kRows; ++k) o[(size_t)k * dim_i] = O[k];
 }
 
+__device__ __forceinline__ int range_begin(int t, int4 lo, int4 hi) {
+    return (t == 0) ? lo.x : (t == 1) ? lo.z : (t == 2) ? hi.x : hi.z;
+}
+__device__ __forceinline__ int range_end(int t, int4 lo, int4 hi) {
+    return (t == 0) ? lo.y : (t == 1) ? lo.w : (t == 2) ? hi.y : hi.w;
+}
+
+// One image: blockIdx.z = orientation.
+template <int kSource, int kMax, int kRows, int kU>
+__global__ __launch_bounds__(256) void k_deskew(const SecDesc* __restrict__ secs, const int* __restrict__ target,
+                                                const float* __restrict__ weight, const uint32_t* __restrict__ fl,
+                                                const float* __restrict__ src, int4 rlo, int4 rhi,
+                                                float* __restrict__ acc, int dimx, int dimy) {
+    const int t = blockIdx.z;
+    deskew_body<kSource, kMax, kRows, kU>(secs, target, weight, fl, src, t, range_begin(t, rlo, rhi),
+                                          range_end(t, rlo, rhi), acc, dimx, dimy);
+}
+
+// Every image of a chunk in one launch: blockIdx.z = 4 * part + orientation,
+// so the images' cells run side by side instead of one launch after another.
+template <int kSource, int kMax, int kRows, int kU>
+__global__ __launch_bounds__(256) void k_deskew_parts(const SecDesc* __restrict__ secs,
+                                                      const int* __restrict__ target,
+                                                      const float* __restrict__ weight,
+                                                      const uint32_t* __restrict__ fl, const float* __restrict__ src,
+                                                      const DeskewPart* __restrict__ parts, float* __restrict__ acc) {
+    const DeskewPart& P = parts[blockIdx.z >> 2];
+    const int t = blockIdx.z & 3;
+    deskew_body<kSource, kMax, kRows, kU>(secs, target, weight, fl, src, t, P.range[2 * t], P.range[2 * t + 1],
+                                          acc + P.acc_off, P.dimx, P.dimy);
+}
+
 // 16 rows, one sector per pass: measured fastest on the batched viewshed
 // (129 VGPRs; 8 rows 1.2x, 4 rows 1.9x, 1 row 4.1x slower, 24 or 32 rows
 // 1.45x slower at 151 / 191 VGPRs; a second sector per pass only adds
-// registers).
-constexpr int kDeskewRows = 16, kDeskewU = 1;
+// registers).  The Radon deskew reads a short vector q per sector (L1/L2
+// resident), so its time is the chain of dependent table and tap reads over
+// the sectors: four sectors per pass.
+constexpr int kDeskewRows = 16, kDeskewU = 1, kDeskewRadonU = 4;
 
 hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source, int is_max,
                          float* acc, int dimx, int dimy, long long img_acc, hipStream_t s) {
@@ -296,9 +328,29 @@ hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src,
     } else if (source == kDeskewArena) {
         if (is_max) SKE_DESKEW(kDeskewArena, 1); else SKE_DESKEW(kDeskewArena, 0);
     } else {
-        SKE_DESKEW(kDeskewRadon, 0);
+        hipLaunchKernelGGL((k_deskew<kDeskewRadon, 0, kDeskewRows, kDeskewRadonU>), grid, dim3(256), 0, s, p.secs,
+                           p.target, p.weight, p.fl, src, lo, hi, a, dimx, dimy);
     }
 #undef SKE_DESKEW
+    return hipGetLastError();
+}
+
+hipError_t launch_deskew_parts(const DevPlan& p, const DeskewPart* parts, int nparts, int max_dim, const float* src,
+                               int source, int is_max, float* acc, hipStream_t s) {
+    if (nparts <= 0) return hipSuccess;
+    if (nparts > 16383) return hipErrorInvalidValue;
+    dim3 grid((max_dim + 255) / 256, (max_dim + kDeskewRows - 1) / kDeskewRows, 4 * nparts);
+#define SKE_DESKEW_P(SRC, MX, U)                                                                                  \
+    hipLaunchKernelGGL((k_deskew_parts<SRC, MX, kDeskewRows, U>), grid, dim3(256), 0, s, p.secs, p.target, p.weight, \
+                       p.fl, src, parts, acc)
+    if (source == kDeskewDense) {
+        if (is_max) SKE_DESKEW_P(kDeskewDense, 1, kDeskewU); else SKE_DESKEW_P(kDeskewDense, 0, kDeskewU);
+    } else if (source == kDeskewArena) {
+        if (is_max) SKE_DESKEW_P(kDeskewArena, 1, kDeskewU); else SKE_DESKEW_P(kDeskewArena, 0, kDeskewU);
+    } else {
+        SKE_DESKEW_P(kDeskewRadon, 0, kDeskewRadonU);
+    }
+#undef SKE_DESKEW_P
     return hipGetLastError();
 }
 

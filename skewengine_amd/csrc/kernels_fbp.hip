@@ -166,7 +166,8 @@ __global__ __launch_bounds__(64 * kPtWaves) void k_fbp_project_tiles(const float
                 const float fu = floorf(u), fv = floorf(v);
                 const float cu = __builtin_amdgcn_fmed3f(fu, fx0, fx1), cv = __builtin_amdgcn_fmed3f(fv, fy0, fy1);
                 const bool in = kin && cu == fu && cv == fv;
-                const float* r = L + (lt + __mul24((int)cv, kPtPitch) + (int)cu);
+                // cv * pitch + cu is an integer below 2^24: exact in float
+                const float* r = L + (lt + (int)__builtin_fmaf(cv, (float)kPtPitch, cu));
                 const float wu = u - fu, wv = v - fv;
                 const float ta = r[0], tb = r[1], tc = r[kPtPitch], td = r[kPtPitch + 1];
                 const float top = __builtin_fmaf(wu, tb - ta, ta);

@@ -173,12 +173,13 @@ __global__ __launch_bounds__(256) void k_radn_rowsum_cols(const SecDesc* __restr
 }
 
 hipError_t launch_radn_rowsum(const DevPlan& p, const float* in4, const float* Sbuf, double* P, hipStream_t s) {
-    if (p.nwork == 0) return hipSuccess;
-    if (!Sbuf && p.nsec > 0 && p.maxH > 0) {
+    if (!Sbuf) {  // writes every row r < H of every sector (rows without cells: 0)
+        if (p.nsec == 0 || p.maxH <= 0) return hipSuccess;
         hipLaunchKernelGGL(k_radn_rowsum_cols, dim3((p.maxH + 255) / 256, p.nsec), dim3(256), 0, s, p.secs, in4,
                            p.target, p.weight, p.fl, P);
         return hipGetLastError();
     }
+    if (p.nwork == 0) return hipSuccess;  // writes the rows of the work list only: P zeroed by the caller
     hipLaunchKernelGGL(k_radn_rowsum, dim3((p.nwork + 3) / 4), dim3(256), 0, s, p.secs, p.work, p.nwork, in4,
                        p.target, p.weight, p.fl, Sbuf, P);
     return hipGetLastError();

@@ -88,6 +88,10 @@ hipError_t launch_radn_smear(const DevPlan& p, const float* q, float* SO, hipStr
 hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source,
                          int is_max, float* acc, int dimx, int dimy, long long img_acc,
                          hipStream_t s);
+// The same for every image of a chunk in one launch: parts[k] holds image k's
+// sector ranges, accumulator offset and sides (max_dim: the largest side).
+hipError_t launch_deskew_parts(const DevPlan& p, const DeskewPart* parts, int nparts, int max_dim, const float* src,
+                               int source, int is_max, float* acc, hipStream_t s);
 
 // reduce(): out (+)= o0, o1, o2, o3 with the inverse orientation maps
 // (src/skewEngine.cuh:885-912).  present[t] == 0 skips buffer t.

@@ -277,7 +277,6 @@ int execute_noalloc(int kernel_id, int n, const float* d_input, float* d_output,
                 SKE_HIP(launch_fill_f64(W, (size_t)ch.ip.so_total, stream));
                 SKE_HIP(launch_lrt(v, in4, nullptr, W, arena, ctab, gtab, stream));
             } else {
-                SKE_HIP(launch_fill_f64(P, (size_t)ch.ip.q_total, stream));
                 SKE_HIP(launch_radn_rowsum(v, in4, nullptr, P, stream));
                 SKE_HIP(launch_radn_filter(v, P, q, tw, tw_off, 1 << next_pow2_log(ch.ip.maxH), stream));
             }

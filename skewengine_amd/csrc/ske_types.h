@@ -39,6 +39,16 @@ struct SecDesc {
     int pad0;
 };
 
+// One image of a chunk for the deskew: its sector range of orientation t
+// ([range[2t], range[2t+1]) indices into the chunk's sectors), the element
+// offset of its four accumulators and its sides.
+struct DeskewPart {
+    int range[8];
+    long long acc_off;
+    int dimx;
+    int dimy;
+};
+
 // One row of one sector: the unit of work of the row kernels.
 struct RowWork {
     int sec;   // index into the SecDesc array
