@@ -519,10 +519,20 @@ struct Core {
             }
             const float* src = radon ? q.p : arena.p;
             const int source = radon ? kDeskewRadon : kDeskewArena;
-            SKE_TRY(launch("deskew", [&] {
-                return launch_deskew_parts(v, cp->dparts.p, (int)cp->parts.size(), cp->max_n, src, source, is_max,
-                                           acc.p, stream);
-            }));
+            if (radon) {  // latency-bound per image: every image of the chunk in one launch
+                SKE_TRY(launch("deskew", [&] {
+                    return launch_deskew_parts(v, cp->dparts.p, (int)cp->parts.size(), cp->max_n, src, source, is_max,
+                                               acc.p, stream);
+                }));
+            } else {  // grids sized to each image (a chunk's images may differ in size)
+                for (const Chunk::Part& part : cp->parts) {
+                    const Image& im = imgs[part.img];
+                    SKE_TRY(launch("deskew", [&] {
+                        return launch_deskew(v, part.range, src, source, is_max, acc.p, im.n, im.n, im.acc_off,
+                                             stream);
+                    }));
+                }
+            }
         }
         const size_t out_n = out.n;
         SKE_TRY(launch("fill", [&] { return launch_fill_f32(out.p, out_n, 0.0f, stream); }));

@@ -343,12 +343,12 @@ hipError_t launch_deskew_parts(const DevPlan& p, const DeskewPart* parts, int np
 #define SKE_DESKEW_P(SRC, MX, U)                                                                                  \
     hipLaunchKernelGGL((k_deskew_parts<SRC, MX, kDeskewRows, U>), grid, dim3(256), 0, s, p.secs, p.target, p.weight, \
                        p.fl, src, parts, acc)
-    if (source == kDeskewDense) {
-        if (is_max) SKE_DESKEW_P(kDeskewDense, 1, kDeskewU); else SKE_DESKEW_P(kDeskewDense, 0, kDeskewU);
-    } else if (source == kDeskewArena) {
+    if (source == kDeskewArena) {
         if (is_max) SKE_DESKEW_P(kDeskewArena, 1, kDeskewU); else SKE_DESKEW_P(kDeskewArena, 0, kDeskewU);
-    } else {
+    } else if (source == kDeskewRadon) {
         SKE_DESKEW_P(kDeskewRadon, 0, kDeskewRadonU);
+    } else {
+        return hipErrorInvalidValue;  // the dense source has one image per call
     }
 #undef SKE_DESKEW_P
     return hipGetLastError();
