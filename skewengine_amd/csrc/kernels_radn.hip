@@ -86,26 +86,29 @@ __global__ __launch_bounds__(256) void k_radn_rowsum(const SecDesc* __restrict__
 // outside a lane's own range are read but add zero (acc + 0 = acc: acc is
 // never -0).
 constexpr int kColsUnroll = 8;
-template <bool kRev>
-__device__ __forceinline__ double rowsum_cols_walk(const int* __restrict__ tt, const float* __restrict__ ww,
-                                                   __amdgpu_buffer_rsrc_t src, int A, int B, int C, int D, int r,
-                                                   int dim_o, int f, int l, int lo, int hi) {
+// Columns [j0, j1) of the walk, adding to acc.  kMask: test the lane's own
+// [f, l) (columns where some lane of the wave is outside its range).  The
+// first tap's value is w * x1 without the reference's 0 + (it differs only
+// in the sign of a zero, which no later sum can see: acc is never -0).
+template <bool kRev, bool kMask>
+__device__ __forceinline__ void rowsum_cols_span(const int* __restrict__ tt, const float* __restrict__ ww,
+                                                 __amdgpu_buffer_rsrc_t src, int A, int B, int C, int D, int r,
+                                                 int dim_o, int f, int l, int j0, int j1, double& acc) {
     auto cell = [&](int j, int t, float w, uint2 pr) -> float {
         // kRev: the tap of row r - t at the lower address
         const float x1 = __uint_as_float(kRev ? pr.y : pr.x), x0 = __uint_as_float(kRev ? pr.x : pr.y);
         const int i0 = r - t;
-        const float s = ((unsigned)(i0 - 1) < (unsigned)dim_o) ? 0.0f + w * x1 : 0.0f;
+        const float s = ((unsigned)(i0 - 1) < (unsigned)dim_o) ? w * x1 : 0.0f;
         const float v = ((unsigned)i0 < (unsigned)dim_o) ? s + (1.0f - w) * x0 : s;
-        return ((unsigned)(j - f) < (unsigned)(l - f)) ? v : 0.0f;
+        return (!kMask || (unsigned)(j - f) < (unsigned)(l - f)) ? v : 0.0f;
     };
     auto load = [&](int j, int t) -> uint2 {
         const int o = A + B * j + C * t + D;
         return make_uint2(__builtin_amdgcn_raw_buffer_load_b32(src, o, 0, 0),
                           __builtin_amdgcn_raw_buffer_load_b32(src, o + 4, 0, 0));
     };
-    double acc = 0.0;
-    int j = lo;
-    for (; j + kColsUnroll <= hi; j += kColsUnroll) {
+    int j = j0;
+    for (; j + kColsUnroll <= j1; j += kColsUnroll) {
         int t[kColsUnroll];
         float w[kColsUnroll];
         uint2 pr[kColsUnroll];
@@ -119,9 +122,25 @@ __device__ __forceinline__ double rowsum_cols_walk(const int* __restrict__ tt, c
 #pragma unroll
         for (int u = 0; u < kColsUnroll; ++u) acc += (double)cell(j + u, t[u], w[u], pr[u]);
     }
-    for (; j < hi; ++j) {
+    for (; j < j1; ++j) {
         const int t = tt[j];
         acc += (double)cell(j, t, ww[j], load(j, t));
+    }
+}
+
+// The walk over [lo, hi): columns [fmax, lmin), inside every lane's range,
+// without the range test.
+template <bool kRev>
+__device__ __forceinline__ double rowsum_cols_walk(const int* __restrict__ tt, const float* __restrict__ ww,
+                                                   __amdgpu_buffer_rsrc_t src, int A, int B, int C, int D, int r,
+                                                   int dim_o, int f, int l, int lo, int hi, int fmax, int lmin) {
+    double acc = 0.0;
+    if (fmax < lmin) {
+        rowsum_cols_span<kRev, true>(tt, ww, src, A, B, C, D, r, dim_o, f, l, lo, fmax, acc);
+        rowsum_cols_span<kRev, false>(tt, ww, src, A, B, C, D, r, dim_o, f, l, fmax, lmin, acc);
+        rowsum_cols_span<kRev, true>(tt, ww, src, A, B, C, D, r, dim_o, f, l, lmin, hi, acc);
+    } else {
+        rowsum_cols_span<kRev, true>(tt, ww, src, A, B, C, D, r, dim_o, f, l, lo, hi, acc);
     }
     return acc;
 }
@@ -145,13 +164,19 @@ __global__ __launch_bounds__(256) void k_radn_rowsum_cols(const SecDesc* __restr
     }
     if (!(live && l > f)) f = l = 0;  // no cell of this lane is added
     int lo = l > f ? f : INT_MAX, hi = l > f ? l : INT_MIN;
+    // [fmax, lmin): inside every lane's range (empty when some lane has none)
+    int fmax = l > f ? f : INT_MAX, lmin = l > f ? l : INT_MIN;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         lo = min(lo, __shfl_xor(lo, off, 64));
         hi = max(hi, __shfl_xor(hi, off, 64));
+        fmax = max(fmax, __shfl_xor(fmax, off, 64));
+        lmin = min(lmin, __shfl_xor(lmin, off, 64));
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     hi = __builtin_amdgcn_readfirstlane(hi);
+    fmax = __builtin_amdgcn_readfirstlane(fmax);
+    lmin = __builtin_amdgcn_readfirstlane(lmin);
     const int dim_o = sd.dim_o, dim_i = sd.dim_i, type = sd.type;
     const int partner = type == 0 ? 2 : type == 1 ? 3 : type == 2 ? 0 : 2;
     // the partner copy as a raw buffer of N floats: a tap outside it (a row
@@ -162,13 +187,14 @@ __global__ __launch_bounds__(256) void k_radn_rowsum_cols(const SecDesc* __restr
     const float* const ww = weight + sd.tab;
     double acc;
     if (type == 0)  // partner row j, position dim_o-1-a: tap r-t at 4 (j dim_o + dim_o-1 + t - r)
-        acc = rowsum_cols_walk<true>(tt, ww, src, 4 * (dim_o - 1), 4 * dim_o, 4, -4 * r, r, dim_o, f, l, lo, hi);
+        acc = rowsum_cols_walk<true>(tt, ww, src, 4 * (dim_o - 1), 4 * dim_o, 4, -4 * r, r, dim_o, f, l, lo, hi, fmax,
+                                     lmin);
     else if (type == 3)  // partner row dim_i-1-j, position dim_o-1-a
         acc = rowsum_cols_walk<true>(tt, ww, src, 4 * ((dim_i - 1) * dim_o + dim_o - 1), -4 * dim_o, 4, -4 * r, r,
-                                     dim_o, f, l, lo, hi);
+                                     dim_o, f, l, lo, hi, fmax, lmin);
     else  // types 1, 2: partner row dim_i-1-j, position a: tap r-t-1 at 4 ((dim_i-1-j) dim_o + r - t - 1)
         acc = rowsum_cols_walk<false>(tt, ww, src, 4 * ((dim_i - 1) * dim_o - 1), -4 * dim_o, -4, 4 * r, r, dim_o,
-                                      f, l, lo, hi);
+                                      f, l, lo, hi, fmax, lmin);
     if (live) P[sd.q + r] = acc;
 }
 
