@@ -200,6 +200,8 @@ __device__ void deskew_cell(const SecDesc* __restrict__ secs, const int* __restr
     *o = O;
 }
 
+constexpr int kDeskewRows = 16;  // output rows per thread
+
 // kRows consecutive output rows per thread: the kRows + 1 taps of a sector
 // are shared by neighbouring rows.  kU sectors per pass: their table reads,
 // then their gathers are issued together, and each row accumulates in
@@ -305,13 +307,109 @@ __global__ __launch_bounds__(256) void k_deskew_parts(const SecDesc* __restrict_
                                           acc + P.acc_off, P.dimx, P.dimy);
 }
 
+// The Radon deskew with the taps staged in LDS.  A Radon skewOutput row r
+// holds q[r] on [first_r, last_r), so a sector's taps for a workgroup's 256
+// columns j0 .. j0+255 and rows i0 .. i0+15 are q and first|last of the rows
+// i0 + t_j .. i0 + t_j + 16: since t is nondecreasing with steps of 0 or 1,
+// the rows i0 + t_{j0} .. i0 + t_{jlast} + 16, at most 272.  A pass stages
+// them for kRadonU sectors (rows past H as empty rows), then every thread
+// reads its 17 taps per sector from LDS: the values so_cell gives, added in
+// ascending sector order, so the result is the k_deskew one bit for bit.
+constexpr int kRadonU = 4, kRadonStage = 288;
+__device__ __forceinline__ void deskew_radon_body(const SecDesc* __restrict__ secs, const int* __restrict__ target,
+                                                  const float* __restrict__ weight,
+                                                  const uint32_t* __restrict__ fl, const float* __restrict__ q,
+                                                  int t, int sb, int se, float* __restrict__ acc, int dimx,
+                                                  int dimy) {
+    constexpr int kRows = kDeskewRows;
+    __shared__ float sq[kRadonU * kRadonStage];
+    __shared__ uint32_t sfl[kRadonU * kRadonStage];
+    if (sb >= se) return;  // uniform
+    const bool tr = (t == 1 || t == 2);
+    const int dim_i = tr ? dimy : dimx, dim_o = tr ? dimx : dimy;
+    const int i0 = blockIdx.y * kRows;
+    const int j0 = blockIdx.x * blockDim.x;
+    if (i0 >= dim_o || j0 >= dim_i) return;  // uniform
+    const int j = min(j0 + (int)threadIdx.x, dim_i - 1);
+    const bool col = j0 + (int)threadIdx.x < dim_i;
+    const int jlast = min(j0 + (int)blockDim.x - 1, dim_i - 1);
+    const size_t N = (size_t)dimx * dimy;
+    float* o = acc + (size_t)t * N + (size_t)i0 * dim_i + j;
+    const int nrow = min(kRows, dim_o - i0);
+    float O[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) O[k] = (col && k < nrow) ? o[(size_t)k * dim_i] : 0.0f;
+    for (int si = sb; si < se; si += kRadonU) {
+        const int nu = min(kRadonU, se - si);
+        __syncthreads();  // the previous pass's reads are done
+        for (int e = threadIdx.x; e < nu * kRadonStage; e += blockDim.x) {
+            const int u = e / kRadonStage, x = e - u * kRadonStage;
+            const SecDesc& sd = secs[si + u];
+            const int rlo = i0 + target[sd.tab + j0];
+            const int len = min(target[sd.tab + jlast] - target[sd.tab + j0] + kRows + 1, kRadonStage);
+            const int r = rlo + x;
+            const bool in = x < len && r < sd.H;
+            sq[e] = in ? q[sd.q + r] : 0.0f;
+            sfl[e] = in ? fl[sd.rows + r] : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kRadonU; ++u) {
+            if (u >= nu) break;  // uniform
+            const SecDesc& sd = secs[si + u];
+            const int tj = target[sd.tab + j];
+            const float wj = weight[sd.tab + j];
+            // stage index of row i0 + tj (tj - t_{j0} <= j - j0 <= 255; the clamp only
+            // guards the stage's bounds)
+            const int base = u * kRadonStage + min(tj - target[sd.tab + j0], kRadonStage - kRows - 1);
+            float v[kRows + 1];
+#pragma unroll
+            for (int m = 0; m <= kRows; ++m) {
+                const uint32_t f = sfl[base + m];
+                v[m] = (j >= fl_first(f) && j < fl_last(f)) ? sq[base + m] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) O[k] = deskew_acc<0>(O[k], v[k], v[k + 1], wj, sd.angle);
+        }
+    }
+    if (col) {
+#pragma unroll
+        for (int k = 0; k < kRows; ++k)
+            if (k < nrow) o[(size_t)k * dim_i] = O[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_deskew_radon(const SecDesc* __restrict__ secs,
+                                                      const int* __restrict__ target,
+                                                      const float* __restrict__ weight,
+                                                      const uint32_t* __restrict__ fl, const float* __restrict__ q,
+                                                      int4 rlo, int4 rhi, float* __restrict__ acc, int dimx,
+                                                      int dimy) {
+    const int t = blockIdx.z;
+    deskew_radon_body(secs, target, weight, fl, q, t, range_begin(t, rlo, rhi), range_end(t, rlo, rhi), acc, dimx,
+                      dimy);
+}
+
+__global__ __launch_bounds__(256) void k_deskew_radon_parts(const SecDesc* __restrict__ secs,
+                                                            const int* __restrict__ target,
+                                                            const float* __restrict__ weight,
+                                                            const uint32_t* __restrict__ fl,
+                                                            const float* __restrict__ q,
+                                                            const DeskewPart* __restrict__ parts,
+                                                            float* __restrict__ acc) {
+    const DeskewPart& P = parts[blockIdx.z >> 2];
+    const int t = blockIdx.z & 3;
+    deskew_radon_body(secs, target, weight, fl, q, t, P.range[2 * t], P.range[2 * t + 1], acc + P.acc_off, P.dimx,
+                      P.dimy);
+}
+
 // 16 rows, one sector per pass: measured fastest on the batched viewshed
 // (129 VGPRs; 8 rows 1.2x, 4 rows 1.9x, 1 row 4.1x slower, 24 or 32 rows
 // 1.45x slower at 151 / 191 VGPRs; a second sector per pass only adds
 // registers).  The Radon deskew reads a short vector q per sector (L1/L2
 // resident), so its time is the chain of dependent table and tap reads over
 // the sectors: four sectors per pass.
-constexpr int kDeskewRows = 16, kDeskewU = 1, kDeskewRadonU = 4;
+constexpr int kDeskewU = 1, kDeskewRadonU = 4;
 
 hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src, int source, int is_max,
                          float* acc, int dimx, int dimy, long long img_acc, hipStream_t s) {
@@ -327,6 +425,9 @@ hipError_t launch_deskew(const DevPlan& p, const int range[8], const float* src,
         if (is_max) SKE_DESKEW(kDeskewDense, 1); else SKE_DESKEW(kDeskewDense, 0);
     } else if (source == kDeskewArena) {
         if (is_max) SKE_DESKEW(kDeskewArena, 1); else SKE_DESKEW(kDeskewArena, 0);
+    } else if ((size_t)grid.x * grid.y * grid.z >= 1024) {
+        hipLaunchKernelGGL(k_deskew_radon, grid, dim3(256), 0, s, p.secs, p.target, p.weight, p.fl, src, lo, hi, a,
+                           dimx, dimy);
     } else {
         hipLaunchKernelGGL((k_deskew<kDeskewRadon, 0, kDeskewRows, kDeskewRadonU>), grid, dim3(256), 0, s, p.secs,
                            p.target, p.weight, p.fl, src, lo, hi, a, dimx, dimy);
@@ -346,7 +447,15 @@ hipError_t launch_deskew_parts(const DevPlan& p, const DeskewPart* parts, int np
     if (source == kDeskewArena) {
         if (is_max) SKE_DESKEW_P(kDeskewArena, 1, kDeskewU); else SKE_DESKEW_P(kDeskewArena, 0, kDeskewU);
     } else if (source == kDeskewRadon) {
-        SKE_DESKEW_P(kDeskewRadon, 0, kDeskewRadonU);
+        // staged taps pay off with four or more workgroups per CU (a batch of
+        // images); a single small image runs its sector chains with four
+        // sectors' gathers in flight instead (one 512^2 image: 50 us against
+        // 87 us staged)
+        if ((size_t)grid.x * grid.y * grid.z >= 1024)
+            hipLaunchKernelGGL(k_deskew_radon_parts, grid, dim3(256), 0, s, p.secs, p.target, p.weight, p.fl, src,
+                               parts, acc);
+        else
+            SKE_DESKEW_P(kDeskewRadon, 0, kDeskewRadonU);
     } else {
         return hipErrorInvalidValue;  // the dense source has one image per call
     }

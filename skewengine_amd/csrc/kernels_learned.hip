@@ -1012,43 +1012,55 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(const float* __restrict__ dz
 // filter and back-projector gradients
 // ---------------------------------------------------------------------------
 
-// Adjoint of k_fbp_backproject: dq[a][k] += scale * sum over pixels of the
-// pixel's linear weight on bin k.  Per 16 x 16 tile and angle, the tile's
-// contributions land in a 48-bin window accumulated in LDS first.
-constexpr int kAdjWin = 48;
-__global__ __launch_bounds__(256) void k_bp_adjoint(const float* __restrict__ dr, float* __restrict__ dq,
-                                                    const float2* __restrict__ cs, FbpGeom g, float scale) {
-    __shared__ float win[kAdjWin];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * T, j0 = blockIdx.x * T;
-    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    const int i = i0 + ty, j = j0 + tx;
+// Adjoint of k_fbp_backproject as a gather: dq[a][k] = sum over the pixels
+// p with floor(kf_p) in {k - 1, k} of p's weight on bin k -- v - w v when
+// floor(kf_p) = k, w v when it is k - 1 (w = kf_p - floor(kf_p), v = scale *
+// dr[p]) -- with kf_p = (x c + y sn) / delta + det_c computed as the
+// back-projection computes it.  One thread per (slice, angle, bin).  It walks
+// the rows (the columns when |c| < |sn|) in ascending order; in each, kf is
+// linear along the row with slope c (sn) per pixel, so the pixels with kf in
+// [k - 1, k + 1) are a run of at most 2 / max(|c|, |sn|) + 1 <= 3.9 whose
+// ends follow from the row's kf at pixel 0; the run, widened by one pixel
+// each side, is tested pixel by pixel.  Fixed order, no atomics.
+// drT: dr transposed per slice, read on column walks so that a wave's
+// neighbouring bins read neighbouring addresses there too.
+__global__ __launch_bounds__(256) void k_bp_adjoint_gather(const float* __restrict__ dr,
+                                                           const float* __restrict__ drT, float* __restrict__ dq,
+                                                           const float2* __restrict__ cs, FbpGeom g, float scale) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y, b = blockIdx.z;
+    if (k >= g.ndet) return;
+    const float2 t2 = cs[a];
+    const float c = t2.x, sn = t2.y;
     const float inv = 1.0f / g.delta;
-    const float x = ((float)j + 0.5f) * g.delta - 1.0f, y = 1.0f - ((float)i + 0.5f) * g.delta;
-    const float v = dr[((size_t)b * g.n + i) * g.n + j] * scale;
-    // tile centre
-    const float xc = ((float)j0 + 0.5f * T) * g.delta - 1.0f, yc = 1.0f - ((float)i0 + 0.5f * T) * g.delta;
-    float* dqb = dq + (size_t)b * g.nang * g.ndet;
-    for (int a = 0; a < g.nang; a++) {
-        const float2 t2 = cs[a];
-        const int base = (int)floorf((xc * t2.x + yc * t2.y) * inv + g.det_c) - kAdjWin / 2;
-        if (threadIdx.x < kAdjWin) win[threadIdx.x] = 0.0f;
-        __syncthreads();
-        const float kf = (x * t2.x + y * t2.y) * inv + g.det_c;
-        const float fk = floorf(kf);
-        const int k0 = (int)fk;
-        const float w = kf - fk;
-        // |k0 - base - 24| <= ceil(T*sqrt(2)/2) + 1 < 24: inside the window
-        atomicAdd(&win[k0 - base], v - w * v);
-        atomicAdd(&win[k0 + 1 - base], w * v);
-        __syncthreads();
-        if (threadIdx.x < kAdjWin) {
-            const int k = base + (int)threadIdx.x;
-            const float s = win[threadIdx.x];
-            if (k >= 0 && k < g.ndet && s != 0.0f) atomicAdd(&dqb[(size_t)a * g.ndet + k], s);
+    const bool rows = fabsf(c) >= fabsf(sn);  // walk rows: kf varies along j with slope c
+    const float* const im = (rows ? dr : drT) + (size_t)b * g.n * g.n;  // line u at im + u n
+    const float slope = rows ? c : -sn;        // d kf / d (index along the walk), about
+    const float rsl = 1.0f / slope;
+    const int cnt = (int)(2.0f * fabsf(rsl)) + 3;  // candidates per line: the run, one more each side
+    float acc = 0.0f;
+    for (int u = 0; u < g.n; u++) {
+        // kf at index 0 of line u (row u: j = 0; column u: i = 0), approximate
+        const float fu = ((float)u + 0.5f) * g.delta;
+        const float kf0 = rows ? ((0.5f * g.delta - 1.0f) * c + (1.0f - fu) * sn) * inv + g.det_c
+                               : ((fu - 1.0f) * c + (1.0f - 0.5f * g.delta) * sn) * inv + g.det_c;
+        // indices t with kf0 + t slope in [k - 1, k + 1)
+        const float ta = ((float)k - 1.0f - kf0) * rsl, tb = ((float)k + 1.0f - kf0) * rsl;
+        const int t0 = (int)floorf(fminf(ta, tb)) - 1;
+        for (int q = 0; q < cnt; q++) {
+            const int t = t0 + q;
+            if (t < 0 || t >= g.n) continue;
+            const int i = rows ? u : t, j = rows ? t : u;
+            const float x = ((float)j + 0.5f) * g.delta - 1.0f, y = 1.0f - ((float)i + 0.5f) * g.delta;
+            const float kf = (x * c + y * sn) * inv + g.det_c;
+            const float fk = floorf(kf);
+            const int k0 = (int)fk;
+            if (k0 != k && k0 != k - 1) continue;
+            const float w = kf - fk;
+            const float v = im[(size_t)u * g.n + t] * scale;
+            acc += (k0 == k) ? v - w * v : w * v;
         }
-        __syncthreads();
     }
+    dq[((size_t)b * g.nang + a) * g.ndet + k] = acc;
 }
 
 // dh[m + ndet - 1] += sum_k dq[row][k] * p[row][k - m] for every row.
@@ -1227,10 +1239,25 @@ hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const flo
     return B.C == 32 ? backward_t<32>(B, prm, dout, grad, n, batch, s) : backward_t<16>(B, prm, dout, grad, n, batch, s);
 }
 
-hipError_t launch_filter_backward(const float* dr0, float* dq, const float* sino, float* dfull, float* dsym,
-                                  const float2* cs, const FbpGeom& g, int batch, hipStream_t s) {
+// out[b][j][i] = in[b][i][j] for n x n slices, through 32 x 32 LDS tiles.
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, float* __restrict__ out, int n) {
+    __shared__ float tile[32][33];
+    const size_t off = (size_t)blockIdx.z * n * n;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8)
+        if (y0 + k < n && x0 + tx < n) tile[k][tx] = in[off + (size_t)(y0 + k) * n + x0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8)
+        if (x0 + k < n && y0 + tx < n) out[off + (size_t)(x0 + k) * n + y0 + tx] = tile[tx][k];
+}
+
+hipError_t launch_filter_backward(const float* dr0, float* drT, float* dq, const float* sino, float* dfull,
+                                  float* dsym, const float2* cs, const FbpGeom& g, int batch, hipStream_t s) {
     const float scale = (float)(M_PI / g.nang);
-    hipLaunchKernelGGL(k_bp_adjoint, dim3(g.n / T, g.n / T, batch), dim3(256), 0, s, dr0, dq, cs, g, scale);
+    hipLaunchKernelGGL(k_transpose, dim3((g.n + 31) / 32, (g.n + 31) / 32, batch), dim3(256), 0, s, dr0, drT, g.n);
+    hipLaunchKernelGGL(k_bp_adjoint_gather, dim3((g.ndet + 255) / 256, g.nang, batch), dim3(256), 0, s, dr0, drT, dq,
+                       cs, g, scale);
     const size_t lds = sizeof(float) * 2 * (size_t)g.ndet;
     hipLaunchKernelGGL(k_filter_grad, dim3((unsigned)((size_t)batch * g.nang)), dim3(256), lds, s, dq, sino, dfull,
                        g.ndet);

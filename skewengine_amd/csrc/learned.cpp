@@ -27,7 +27,7 @@ struct ske_learned {
     bool derived = false;          // full taps and bf16 weights match prm (cleared whenever prm changes)
     DevBuf<float> prm, grad, m1, m2, full, dq, dfull, dout;
     DevBuf<double> loss;
-    DevBuf<float> r0, dz, dr0;     // dz: three layers back to back
+    DevBuf<float> r0, dz, dr0, drT;  // dz: three layers back to back; drT: dr0 transposed
     DevBuf<uint16_t> act, wb;      // act: a1, a2, a3 back to back; wb: bf16 weights (learned_wb_size)
     DevBuf<float> ell;
     size_t pix(int batch) const { return (size_t)f->g.n * f->g.n * (size_t)batch; }
@@ -130,6 +130,7 @@ int ensure_train(ske_learned* l) {
     const FbpGeom& g = l->f->g;
     SKE_TRY(l->dz.ensure(3 * l->pix(mb) * l->C));
     SKE_TRY(l->dr0.ensure(l->pix(mb)));
+    SKE_TRY(l->drT.ensure(l->pix(mb)));
     SKE_TRY(l->dout.ensure(l->pix(mb)));
     SKE_TRY(l->dq.ensure((size_t)mb * g.nang * g.ndet));
     SKE_TRY(l->dfull.ensure(2 * (size_t)g.ndet - 1));
@@ -152,12 +153,10 @@ int gradient(ske_learned* l, int batch) {
         return launch_cnn_backward(B, l->prm.p, l->dout.p, l->grad.p, g.n, batch, s);
     }));
     return f->timer.launch(s, "filter_backward", [&] {
-        hipError_t e = launch_fill_f32(l->dq.p, (size_t)batch * g.nang * g.ndet, 0.0f, s);
+        hipError_t e = launch_fill_f32(l->dfull.p, 2 * (size_t)g.ndet - 1, 0.0f, s);
         if (e != hipSuccess) return e;
-        e = launch_fill_f32(l->dfull.p, 2 * (size_t)g.ndet - 1, 0.0f, s);
-        if (e != hipSuccess) return e;
-        return launch_filter_backward(l->dr0.p, l->dq.p, f->buf[SKE_FBP_SINO].p, l->dfull.p, l->grad.p + l->L.Taps,
-                                      f->cs.p, g, batch, s);
+        return launch_filter_backward(l->dr0.p, l->drT.p, l->dq.p, f->buf[SKE_FBP_SINO].p, l->dfull.p,
+                                      l->grad.p + l->L.Taps, f->cs.p, g, batch, s);
     });
 }
 

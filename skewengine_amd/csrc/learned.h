@@ -60,8 +60,11 @@ hipError_t launch_cnn_forward(const LearnedBufs& B, const float* prm, int n, int
 hipError_t launch_cnn_infer(const LearnedBufs& B, const float* prm, int n, int batch, hipStream_t s);
 hipError_t launch_cnn_backward(const LearnedBufs& B, const float* prm, const float* dout, float* grad, int n,
                                int batch, hipStream_t s);
-hipError_t launch_filter_backward(const float* dr0, float* dq, const float* sino, float* dfull, float* dsym,
-                                  const float2* cs, const FbpGeom& g, int batch, hipStream_t s);
+// dq = the back-projection's adjoint of dr0 (written, not added; drT: batch
+// n x n floats of scratch for dr0 transposed), then the filter gradient into
+// dfull and its fold into the symmetric taps dsym.
+hipError_t launch_filter_backward(const float* dr0, float* drT, float* dq, const float* sino, float* dfull,
+                                  float* dsym, const float2* cs, const FbpGeom& g, int batch, hipStream_t s);
 hipError_t launch_taps_expand(const float* sym, float* full, int ndet, hipStream_t s);
 hipError_t launch_adam(float* prm, const float* grad, float* m1, float* m2, int count, float lr, float lr_taps,
                        int taps_off, int step, hipStream_t s);
