@@ -662,7 +662,9 @@ struct Core {
     }
 
     // Time candidate viewshed launch shapes on a sample (one grid per distinct
-    // side, every 12th sector) and keep the fastest.
+    // side, every 12th sector): a warm-up launch, then the best of three timed
+    // ones per shape; keep the first shape in list order within 2% of the
+    // fastest, so that shapes a box's noise cannot separate give one answer.
     int autotune_sdem(float povh, int* waves, int* wgs, int* occ, double* best_ms) {
         SKE_HIP(hipSetDevice(device));
         std::unique_ptr<DevPlanBuf> sample(new DevPlanBuf());
@@ -678,26 +680,35 @@ struct Core {
         DevBuf<float> scratch;
         SKE_TRY(scratch.ensure((size_t)sample->ip.so_total));
         const DevPlan v = sample->view();
-        const SdemTuning cands[] = {{4, 0, 0, 1}, {4, 8, 8, 1}, {4, 16, 8, 1}, {8, 8, 8, 1}, {2, 16, 8, 1},
-                                    {4, 0, 0, 4}, {8, 0, 0, 4}, {2, 0, 0, 4}, {4, 8, 6, 4}, {4, 8, 0, 4}};
+        // Listed with the shape that measures fastest on the 180-sector batches first.
+        const SdemTuning cands[] = {{4, 0, 0, 4}, {4, 0, 0, 1}, {8, 0, 0, 4}, {2, 0, 0, 4}, {4, 8, 6, 4},
+                                    {4, 8, 0, 4}, {4, 8, 8, 1}, {4, 16, 8, 1}, {8, 8, 8, 1}, {2, 16, 8, 1}};
         hipEvent_t a = take_event(), b = take_event();
+        constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
+        double t[kCands];
         double best = 1e300;
-        SdemTuning pick = tune;
-        for (const SdemTuning& c : cands) {  // a warm-up launch, then the best of two timed ones
+        for (int ci = 0; ci < kCands; ci++) {
+            const SdemTuning& c = cands[ci];
             SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, nullptr, stream));
-            for (int rep = 0; rep < 2; rep++) {
+            t[ci] = 1e300;
+            for (int rep = 0; rep < 3; rep++) {
                 SKE_HIP(hipEventRecord(a, stream));
                 SKE_HIP(launch_op_sdem_tuned(v, in4.p, nullptr, scratch.p, rtab.p, povh, c, counter.p, nullptr, stream));
                 SKE_HIP(hipEventRecord(b, stream));
                 SKE_HIP(hipEventSynchronize(b));
                 float ms = 0.0f;
                 SKE_HIP(hipEventElapsedTime(&ms, a, b));
-                if (ms < best) {
-                    best = ms;
-                    pick = c;
-                }
+                t[ci] = std::min(t[ci], (double)ms);
             }
+            best = std::min(best, t[ci]);
         }
+        SdemTuning pick = tune;
+        for (int ci = 0; ci < kCands; ci++)
+            if (t[ci] <= 1.02 * best) {
+                pick = cands[ci];
+                best = t[ci];
+                break;
+            }
         event_pool.push_back(a);
         event_pool.push_back(b);
         SKE_HIP(hipStreamSynchronize(stream));
