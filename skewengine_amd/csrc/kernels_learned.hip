@@ -48,9 +48,11 @@ constexpr int T = 16;  // tile side
 // forward
 // ---------------------------------------------------------------------------
 
-// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout and transposed
-// ([ci][t][co], the backward's), and W1 as bf16 [co][16] with k = 4*ky + kx
-// (kx = 3 and k >= 12 zero): the matrix-core operands.
+// W2, W3 fp32 [co][t][ci] -> bf16 copies in the same layout (the forward's,
+// each 8-channel chunk in the stored channel order 0, 1, 4, 5, 2, 3, 6, 7 of
+// the activation tiles) and transposed ([ci][t][co], the backward's), and W1
+// as bf16 [co][16] with k = 4*ky + kx (kx = 3 and k >= 12 zero): the
+// matrix-core operands.
 template <int C>
 __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restrict__ wb) {
     constexpr LearnedLayout L = learned_layout(C);
@@ -63,8 +65,10 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
     if (i < 9 * C * C) {
         const int co = i / (9 * C), t = (i / C) % 9, ci = i % C;
         const uint16_t v2 = f2bf(prm[L.W2 + i]), v3 = f2bf(prm[L.W3 + i]);
-        wb2[i] = v2;
-        wb3[i] = v3;
+        const int pw = (ci >> 1) & 3;                                     // bf16 pair within the chunk
+        const int st = i - ci + (ci & ~7) + 2 * ((pw == 1 || pw == 2) ? 3 - pw : pw) + (ci & 1);
+        wb2[st] = v2;
+        wb3[st] = v3;
         wt2[(ci * 9 + t) * C + co] = v2;
         wt3[(ci * 9 + t) * C + co] = v3;
     }
@@ -148,6 +152,17 @@ struct MidK {
     static constexpr int kHalves = C / 16;                           // 16-channel output blocks
 };
 
+// Waves of a forward workgroup: kFWaves row bands; for C = 32 each band's two
+// 16-channel output blocks go to two waves, which halves the weights and
+// accumulators a wave holds (twice as many waves per SIMD hide the operand
+// reads' latency, for twice the operand reads).
+template <int C>
+struct FwdSplit {
+    static constexpr int kSplit = C == 32 ? 2 : 1;         // waves per band
+    static constexpr int kWaves = kFWaves * kSplit;        // waves per workgroup
+    static constexpr int kHW = MidK<C>::kHalves / kSplit;  // channel blocks per wave
+};
+
 // The K blocks as reads of a sweep down the rows.  Block b of output row y
 // multiplies the operand of "kind" kind(b) read at input row y + dy(b); lane
 // block kb of a kind-k read at row r takes the pixel (r + krow(k, kb),
@@ -182,14 +197,14 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast
 
 // B operand of conv 1 -> C (v_mfma_f32_16x16x16_bf16, k = 4*ky + kx) for an
 // a1-region pixel (y, x): lane block kb < 3 holds the bf16 pairs (x, x+1),
-// (x+2, x+3) of tile row y + kb, block 3 zeros.  `p` points at the lane's
-// pair in row y + min(kb, 2) of the copy packed at the parity of x (the
-// even-column copy at s0, the odd one at s0 + kS0Odd); `off` adds whole
-// rows; `m` is all ones for kb < 3, else zero.
+// (x+2, x+3) of tile row y + kb, and block 3 those of row y + 2 again
+// (finite values that meet the zero weights of k >= 12).  `p` points at the
+// lane's pair in row y + min(kb, 2) of the copy packed at the parity of x
+// (the even-column copy at s0, the odd one at s0 + kS0Odd); `off` adds
+// whole rows.  The pair of words is 4-byte aligned only: two 32-bit reads.
 typedef short s16x4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ s16x4 in_operand(const uint32_t* p, int off, uint32_t m) {
-    const uint2 v = make_uint2(p[off] & m, p[off + 1] & m);
-    return __builtin_bit_cast(s16x4, v);
+__device__ __forceinline__ s16x4 in_operand(const uint32_t* p, int off) {
+    return __builtin_bit_cast(s16x4, make_uint2(p[off], p[off + 1]));
 }
 
 // relu, bf16, and a lane's 4 channels of one pixel (8 bytes); `keep` false
@@ -231,22 +246,22 @@ __device__ __forceinline__ void store_block(uint16_t* dst, const f32x4& acc, boo
 __device__ __forceinline__ uint4 pair_swap(uint4 v) { return make_uint4(v.x, v.z, v.y, v.w); }
 
 // A operands of a conv C -> C layer: lane (co = l%16, kb = l/16) of channel
-// block h holds W[16h + co][tap][ch .. ch+7] for K block b, with the lane's
-// tap of Sweep<C> and ch = 8*(kb % (C/8)) (zeros in block 4's first half
-// for C = 16).
-template <int C>
-__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
-                                                 const uint16_t* __restrict__ wb, int co, int kb) {
+// block h0 + h holds W[16(h0 + h) + co][tap][ch .. ch+7] for K block b, with
+// the lane's tap of Sweep<C> and ch = 8*(kb % (C/8)) (zeros in block 4's
+// first half for C = 16), in the stored channel order, as packed.
+template <int C, int HW>
+__device__ __forceinline__ void load_mid_weights(bf16x8 (&w)[HW][MidK<C>::kBlocks], const uint16_t* __restrict__ wb,
+                                                 int co, int kb, int h0) {
     using K = MidK<C>;
 #pragma unroll
-    for (int h = 0; h < K::kHalves; h++)
+    for (int h = 0; h < HW; h++)
 #pragma unroll
         for (int b = 0; b < K::kBlocks; b++) {
             int ty, tx;
             bool zero;
             Sweep<C>::tap(b, kb, ty, tx, zero);
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (!zero) v = pair_swap(*(const uint4*)(wb + ((16 * h + co) * 9 + 3 * ty + tx) * C + 8 * (kb % K::kChunks)));
+            if (!zero) v = *(const uint4*)(wb + ((16 * (h0 + h) + co) * 9 + 3 * ty + tx) * C + 8 * (kb % K::kChunks));
             w[h][b] = as_bf16x8(v);
         }
 }
@@ -266,17 +281,18 @@ struct MidLeft {
 // `kin` at band row 0, rows as immediate offsets) and multiplies into the
 // output rows r, r - 1, r - 2 it feeds; output row y is complete, and
 // stored, once row y + 2 has been read.  yimg: the image row of band row 0.
-template <int C, bool kEdge, int WI, int PO, int R>
+// HW channel blocks, stored at rout + h * pout.
+template <int C, int HW, bool kEdge, int WI, int PO, int R>
 __device__ __forceinline__ void mid_sweep(const uint16_t* const (&kin)[Sweep<C>::kKinds], uint16_t* rout, int pout,
-                                          const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
-                                          const f32x4 (&bias)[MidK<C>::kHalves], int yimg, int n, bool xin) {
+                                          const bf16x8 (&w)[HW][MidK<C>::kBlocks], const f32x4 (&bias)[HW],
+                                          int yimg, int n, bool xin) {
     using K = MidK<C>;
     using S = Sweep<C>;
-    f32x4 acc[R][K::kHalves];
+    f32x4 acc[R][HW];
 #pragma unroll
     for (int y = 0; y < R; y++)
 #pragma unroll
-        for (int h = 0; h < K::kHalves; h++) acc[y][h] = bias[h];
+        for (int h = 0; h < HW; h++) acc[y][h] = bias[h];
 #pragma unroll
     for (int r = 0; r < R + 2; r++) {
 #pragma unroll
@@ -292,7 +308,7 @@ __device__ __forceinline__ void mid_sweep(const uint16_t* const (&kin)[Sweep<C>:
                 const int y = r - S::dy(b);
                 if (S::kind(b) != k || y < 0 || y >= R) continue;
 #pragma unroll
-                for (int h = 0; h < K::kHalves; h++)
+                for (int h = 0; h < HW; h++)
                     acc[y][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][b], v, acc[y][h], 0, 0, 0);
             }
         }
@@ -300,7 +316,7 @@ __device__ __forceinline__ void mid_sweep(const uint16_t* const (&kin)[Sweep<C>:
             const int y = r - 2;
             const bool keep = xin && (!kEdge || (yimg + y >= 0 && yimg + y < n));
 #pragma unroll
-            for (int h = 0; h < K::kHalves; h++) store_block(rout + h * pout + y * PO * 16, acc[y][h], keep, true);
+            for (int h = 0; h < HW; h++) store_block(rout + h * pout + y * PO * 16, acc[y][h], keep, true);
         }
     }
 }
@@ -311,10 +327,12 @@ __device__ __forceinline__ void mid_sweep(const uint16_t* const (&kin)[Sweep<C>:
 // first 16 columns of each row -- wave w sweeps a band of consecutive rows
 // (side 20: five rows each; side 18: four, four, five, five) -- then the
 // leftover columns, whose groups read every block's operand themselves.
-template <int C, bool kEdge, int WI, int PO>
+// The wave computes HW channel blocks, stored from `out` (the region of its
+// first block) at plane stride pout; `wave` is its row band.
+template <int C, int HW, bool kEdge, int WI, int PO>
 __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t* out, int pout, int oy, int ox,
-                                          int n, const bf16x8 (&w)[MidK<C>::kHalves][MidK<C>::kBlocks],
-                                          const f32x4 (&bias)[MidK<C>::kHalves], int wave, int j16, int kb) {
+                                          int n, const bf16x8 (&w)[HW][MidK<C>::kBlocks], const f32x4 (&bias)[HW],
+                                          int wave, int j16, int kb) {
     using K = MidK<C>;
     using S = Sweep<C>;
     constexpr int WO = WI - 2;
@@ -330,9 +348,9 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
         uint16_t* const rout = out + chunk_off(y0, j16, PO, kb >> 1) + 4 * (kb & 1);
         const bool xin = !kEdge || (ox + j16 >= 0 && ox + j16 < n);
         if (five)
-            mid_sweep<C, kEdge, WI, PO, 5>(kin, rout, pout, w, bias, oy + y0, n, xin);
+            mid_sweep<C, HW, kEdge, WI, PO, 5>(kin, rout, pout, w, bias, oy + y0, n, xin);
         else
-            mid_sweep<C, kEdge, WI, PO, 4>(kin, rout, pout, w, bias, oy + y0, n, xin);
+            mid_sweep<C, HW, kEdge, WI, PO, 4>(kin, rout, pout, w, bias, oy + y0, n, xin);
     }
     {   // leftover columns: lane j's pixel row is kRows * g + ro(j); only the
         // 18-wide layer's last group has rows past the side (read clamped, not stored)
@@ -355,22 +373,21 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
             if (wave + kFWaves * mm >= M::kGroups) break;  // wave-uniform
             const int y = y0 + kFWaves * M::kRows * mm;
             constexpr int kStep = kFWaves * M::kRows;      // rows between a lane's groups
-            f32x4 acc[K::kHalves];
+            f32x4 acc[HW];
 #pragma unroll
-            for (int h = 0; h < K::kHalves; h++) acc[h] = bias[h];
+            for (int h = 0; h < HW; h++) acc[h] = bias[h];
 #pragma unroll
             for (int i = 0; i < K::kBlocks; i++) {
                 const int b = S::order(i);
                 const bf16x8 v = as_bf16x8(*(const uint4*)(rin[b] + mm * kStep * WI * 16));
 #pragma unroll
-                for (int h = 0; h < K::kHalves; h++)
+                for (int h = 0; h < HW; h++)
                     acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[h][b], v, acc[h], 0, 0, 0);
             }
             const int iy = oy + y, ix = ox + x;
             const bool keep = !kEdge || (iy >= 0 && iy < n && ix >= 0 && ix < n);
 #pragma unroll
-            for (int h = 0; h < K::kHalves; h++)
-                store_block(o + h * pout + mm * kStep * PO * 16, acc[h], keep, y < WO);
+            for (int h = 0; h < HW; h++) store_block(o + h * pout + mm * kStep * PO * 16, acc[h], keep, y < WO);
         }
     }
 }
@@ -381,13 +398,11 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
 // columns of each row (wave w: rows w, w + 4, ..., two at a time), then the
 // columns 16..21 as blocks of 2 rows x 8 columns (lanes of columns 22 and
 // 23 idle).  A lane's operand and store addresses are fixed; rows are
-// immediate offsets.
-template <int C, bool kEdge>
-__device__ __forceinline__ void conv_in(const uint32_t* s0, uint16_t* sa, const s16x4 (&w1)[C / 16],
-                                        const f32x4 (&bias1)[C / 16], int i0, int j0, int n, int wave, int j16,
-                                        int kb) {
-    constexpr int H = C / 16;
-    const uint32_t m = kb < 3 ? 0xffffffffu : 0u;
+// immediate offsets.  The wave computes H channel blocks, stored from `sa`
+// (the region of its first block).
+template <int C, int H, bool kEdge>
+__device__ __forceinline__ void conv_in(const uint32_t* s0, uint16_t* sa, const s16x4 (&w1)[H],
+                                        const f32x4 (&bias1)[H], int i0, int j0, int n, int wave, int j16, int kb) {
     const int ry = min(kb, 2);  // tile row offset of this lane's operand
     auto base = [&](int x) { return ((x & 1) ? s0 + kS0Odd : s0) + (x >> 1); };
     {   // first 16 columns
@@ -399,7 +414,7 @@ __device__ __forceinline__ void conv_in(const uint32_t* s0, uint16_t* sa, const 
             constexpr int NR = decltype(nr_tag)::value;
             s16x4 v[NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++) v[r] = in_operand(p, 4 * (r0 + r) * kS0, m);
+            for (int r = 0; r < NR; r++) v[r] = in_operand(p, 4 * (r0 + r) * kS0);
             f32x4 acc[NR][H];
 #pragma unroll
             for (int r = 0; r < NR; r++)
@@ -431,7 +446,7 @@ __device__ __forceinline__ void conv_in(const uint32_t* s0, uint16_t* sa, const 
         for (int mm = 0; mm < (kF1 / 2 + kFWaves - 1) / kFWaves; mm++) {
             if (wave + kFWaves * mm >= kF1 / 2) break;  // wave-uniform
             const int y = 2 * wave + r + 8 * mm;
-            const s16x4 v = in_operand(p, 8 * mm * kS0, m);
+            const s16x4 v = in_operand(p, 8 * mm * kS0);
             const bool keep = xin && (!kEdge || (i0 - 3 + y >= 0 && i0 - 3 + y < n));
             f32x4 acc[H];
 #pragma unroll
@@ -449,7 +464,7 @@ template <int C>
 __device__ __forceinline__ void store_centre(const uint16_t* reg, int pl, int wi, int off,
                                              uint16_t* __restrict__ dst, int i0, int j0, int n) {
     constexpr int kV = C / 8;  // 16-byte vectors per pixel
-    for (int t = threadIdx.x; t < kFT * kFT * kV; t += 64 * kFWaves) {
+    for (int t = threadIdx.x; t < kFT * kFT * kV; t += 64 * FwdSplit<C>::kWaves) {
         const int px = t / kV, h = t % kV;  // h: channels 8h .. 8h+7 (plane h / 2, half h % 2)
         const int y = px / kFT, x = px % kFT;
         const uint4 v = pair_swap(*(const uint4*)(reg + (h >> 1) * pl + chunk_off(y + off, x + off, wi, h & 1)));
@@ -469,27 +484,31 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
                                          uint16_t* sa, uint16_t* sb, uint32_t* s0, uint32_t* w4,
                                          float* red) {
     using K = MidK<C>;
-    constexpr int H = K::kHalves;
+    using F = FwdSplit<C>;
+    constexpr int H = F::kHW;  // channel blocks of this wave
     constexpr LearnedLayout L = learned_layout(C);
     const int b = blockIdx.z;
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const float* img = r0 + (size_t)b * n * n;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar: wave-uniform tests branch
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar: wave-uniform tests branch
+    const int wave = F::kSplit > 1 ? wid % kFWaves : wid;     // row band
+    const int h0 = F::kSplit > 1 ? (wid / kFWaves) * H : 0;  // first channel block
     const int j16 = lane & 15, kb = lane >> 4;  // A row / B column, and k block
     // prologue loads in flight together: W2 operands, W1, biases, W4, r0 tile
     bf16x8 w[H][K::kBlocks];
-    load_mid_weights<C>(w, wb2, j16, kb);
+    load_mid_weights<C, H>(w, wb2, j16, kb, h0);
     s16x4 w1[H];
     f32x4 bias1[H], bias2[H], bias3[H];
 #pragma unroll
     for (int h = 0; h < H; h++) {
-        w1[h] = __builtin_bit_cast(s16x4, *(const uint2*)(w1b + (16 * h + j16) * 16 + 4 * kb));  // W1[co][ky = kb][0..3]
+        const int co = 16 * (h0 + h);
+        w1[h] = __builtin_bit_cast(s16x4, *(const uint2*)(w1b + (co + j16) * 16 + 4 * kb));  // W1[co][ky = kb][0..3]
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            bias1[h][r] = prm[L.B1 + 16 * h + 4 * kb + r];
-            bias2[h][r] = prm[L.B2 + 16 * h + 4 * kb + r];
-            bias3[h][r] = prm[L.B3 + 16 * h + 4 * kb + r];
+            bias1[h][r] = prm[L.B1 + co + 4 * kb + r];
+            bias2[h][r] = prm[L.B2 + co + 4 * kb + r];
+            bias3[h][r] = prm[L.B3 + co + 4 * kb + r];
         }
     }
     {   // the 24 x 24 r0 tile as 4-column quads: thread t < 144 loads quad
@@ -511,37 +530,40 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
             *(uint2*)(s0 + kS0Odd + row * kS0 + 2 * q) = make_uint2(pack_bf16(v.y, v.z), pack_bf16(v.w, e));
         }
     }
-    for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * kFWaves)
-        w4[t] = pack_bf16(prm[L.W4 + 2 * t], prm[L.W4 + 2 * t + 1]);
+    for (int t = threadIdx.x; t < 9 * C / 2; t += 64 * F::kWaves) {  // in the stored channel order
+        const int q = t & 3;
+        w4[t - q + ((q == 1 || q == 2) ? 3 - q : q)] = pack_bf16(prm[L.W4 + 2 * t], prm[L.W4 + 2 * t + 1]);
+    }
     __syncthreads();
-    conv_in<C, kEdge>(s0, sa, w1, bias1, i0, j0, n, wave, j16, kb);
+    conv_in<C, H, kEdge>(s0, sa + h0 * kPlA, w1, bias1, i0, j0, n, wave, j16, kb);
     __syncthreads();
     const size_t img_off = (size_t)b * n * n * C;
     if (kTrain) store_centre<C>(sa, kPlA, kF1, 3, a1g + img_off, i0, j0, n);
-    mid_layer<C, kEdge, kF1, kF2>(sa, kPlA, sb, kPlB, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
-    load_mid_weights<C>(w, wb3, j16, kb);  // in flight while the other waves finish
+    mid_layer<C, H, kEdge, kF1, kF2>(sa, kPlA, sb + h0 * kPlB, kPlB, i0 - 2, j0 - 2, n, w, bias2, wave, j16, kb);
+    load_mid_weights<C, H>(w, wb3, j16, kb, h0);  // in flight while the other waves finish
     __syncthreads();
     if (kTrain) store_centre<C>(sb, kPlB, kF2, 2, a2g + img_off, i0, j0, n);
-    mid_layer<C, kEdge, kF2, kP3>(sb, kPlB, sa, kPlA, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
+    mid_layer<C, H, kEdge, kF2, kP3>(sb, kPlB, sa + h0 * kPlA, kPlA, i0 - 1, j0 - 1, n, w, bias3, wave, j16, kb);
     __syncthreads();
     if (kTrain) store_centre<C>(sa, kPlA, kP3, 1, a3g + img_off, i0, j0, n);
     // conv C -> 1 + bias + residual on the matrix cores: the K blocks and
     // operand reads of the mid layers over the a3 region, with W4 in every
     // row of A, so every row of D is the output; row 0 (lanes 0..15,
-    // register 0) is used.  Wave w sweeps output rows 4w .. 4w + 3.
+    // register 0) is used.  Wave w of the workgroup's W sweeps output rows
+    // R w .. R w + R - 1, R = 16 / W.
     float dsum = 0.0f;  // training: this thread's squared errors
     {
         using S = Sweep<C>;
-        constexpr int R = 4;
+        constexpr int R = kFT / F::kWaves;
         const int ch = kb % K::kChunks;
-        const int y0 = R * wave;
+        const int y0 = R * wid;
         bf16x8 w4a[K::kBlocks];
 #pragma unroll
         for (int q = 0; q < K::kBlocks; q++) {
             int ty, tx;
             bool zero;
             S::tap(q, kb, ty, tx, zero);
-            const uint4 wv = pair_swap(*(const uint4*)(w4 + (3 * ty + tx) * (C / 2) + 4 * ch));  // W4[tap][8ch .. 8ch+7]
+            const uint4 wv = *(const uint4*)(w4 + (3 * ty + tx) * (C / 2) + 4 * ch);  // W4[tap][8ch .. 8ch+7]
             w4a[q] = as_bf16x8(zero ? make_uint4(0, 0, 0, 0) : wv);
         }
         const uint16_t* kin[S::kKinds];
@@ -585,7 +607,7 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
     if (kTrain && target) {  // uniform
         red[threadIdx.x] = dsum;
         __syncthreads();
-        for (int st = 32 * kFWaves; st > 0; st >>= 1) {
+        for (int st = 32 * F::kWaves; st > 0; st >>= 1) {
             if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
             __syncthreads();
         }
@@ -595,9 +617,10 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
 
 
 // C = 16: at most 96 VGPRs, so the five workgroups per CU that LDS allows
-// (five waves per SIMD) also fit the registers.
+// (five waves per SIMD) also fit the registers.  C = 32: eight waves, at
+// most 128 VGPRs, two workgroups per CU (LDS): four waves per SIMD.
 template <int C, bool kTrain>
-__global__ __launch_bounds__(64 * kFWaves, C == 16 ? 5 : 2) void k_cnn_fused_fwd(const float* __restrict__ r0,
+__global__ __launch_bounds__(64 * FwdSplit<C>::kWaves, C == 16 ? 5 : 2) void k_cnn_fused_fwd(const float* __restrict__ r0,
                                                                 const float* __restrict__ prm,
                                                                 const uint16_t* __restrict__ w1b,
                                                                 const uint16_t* __restrict__ wb2,
@@ -612,7 +635,7 @@ __global__ __launch_bounds__(64 * kFWaves, C == 16 ? 5 : 2) void k_cnn_fused_fwd
     __shared__ __align__(16) uint16_t sb[(C / 16) * kPlB];  // a2 (planar)
     __shared__ __align__(16) uint32_t s0[kS0Odd + kF0 * kS0];             // r0 as bf16 pairs at even, then odd columns
     __shared__ __align__(16) uint32_t w4[9 * C / 2];      // W4 as bf16 pairs
-    __shared__ float red[64 * kFWaves];
+    __shared__ float red[64 * FwdSplit<C>::kWaves];
     const int i0 = blockIdx.y * kFT, j0 = blockIdx.x * kFT;
     const bool interior = i0 >= 4 && i0 + kFT + 4 <= n && j0 >= 4 && j0 + kFT + 4 <= n;  // uniform
     if (interior)
@@ -1186,7 +1209,8 @@ template <int C, bool kTrain>
 hipError_t fused_t(const LearnedBufs& B, const float* prm, int n, int batch, const float* target, float* dout,
                    double* loss, hipStream_t s) {
     const float inv = 1.0f / ((float)n * n * batch);
-    hipLaunchKernelGGL((k_cnn_fused_fwd<C, kTrain>), dim3(n / kFT, n / kFT, batch), dim3(64 * kFWaves), 0, s, B.r0,
+    hipLaunchKernelGGL((k_cnn_fused_fwd<C, kTrain>), dim3(n / kFT, n / kFT, batch),
+                       dim3(64 * FwdSplit<C>::kWaves), 0, s, B.r0,
                        prm, B.w1b, B.wb2, B.wb3, B.out, n, kTrain ? B.a1 : nullptr, kTrain ? B.a2 : nullptr,
                        kTrain ? B.a3 : nullptr, target, dout, inv, loss);
     return hipGetLastError();
