@@ -90,7 +90,8 @@ __global__ void k_pack_weights(const float* __restrict__ prm, uint16_t* __restri
 // Matrix cores: v_mfma_f32_16x16x32_bf16 computes D[co][px] += sum_k
 // W[co][k] X[k][px] for a group of 16 pixels and a block of 16 output
 // channels (fp32 accumulate, the bias as the initial accumulator); C = 32
-// runs two channel blocks on the same pixel operand.  Lane layout: A[i][k]
+// has two channel blocks, on two waves that read the same pixel operands
+// (FwdSplit).  Lane layout: A[i][k]
 // at lane i + 16*(k/8), element k%8; B[k][j] at lane j + 16*(k/8), element
 // k%8; D[i][j] at lane j + 16*(i/4), register i%4.
 //   conv 1 -> C: v_mfma_f32_16x16x16_bf16 (4 k per lane) with k = 4*ky + kx
@@ -395,7 +396,7 @@ __device__ __forceinline__ void mid_layer(const uint16_t* in, int pin, uint16_t*
 // conv 1 -> C on the 22 x 22 a1 region at image pixel (i0-3, j0-3), from
 // the packed r0 tile s0, on v_mfma_f32_16x16x16_bf16 (the 12 products of a
 // 3 x 3 window and 4 zeros per pixel).  Groups of 16 pixels: the first 16
-// columns of each row (wave w: rows w, w + 4, ..., two at a time), then the
+// columns of each row (row band w: rows w, w + 4, ..., two at a time), then the
 // columns 16..21 as blocks of 2 rows x 8 columns (lanes of columns 22 and
 // 23 idle).  A lane's operand and store addresses are fixed; rows are
 // immediate offsets.  The wave computes H channel blocks, stored from `sa`
