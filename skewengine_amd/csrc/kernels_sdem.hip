@@ -409,15 +409,21 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
         }
         const int db = d0 + 32;
         nblk += ns;
+        // The exit bound is tested after every other block (blocks 0, 2, 4,
+        // ...), lanes past their row end retire after every block.  A scan
+        // whose bound holds only runs further invisible steps, which leave A
+        // as it is, so its cv does not change; the bound costs about a
+        // seventh of a block's instructions.
+        const bool tb = ((d0 >> 5) & 1) == 0;
         int T = 0;
 #pragma unroll
         for (int k = 0; k < kG; ++k) {
             if (k < ns) {
                 const bool a = (act[k] >> lane) & 1;
                 const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
+                bool done = myLen <= db;
                 // active lanes and empty lanes (jj = safe) keep the bound's table reads in range
-                const bool done =
-                    (myLen <= db) | (exit_bound<kDir, false>(tab, jj[k], L, h[k], db, ytab, invL) <= M[k]);
+                if (tb) done = done | (exit_bound<kDir, false>(tab, jj[k], L, h[k], db, ytab, invL) <= M[k]);
                 if (a && done) {
                     const int cv = myLen > db ? A[k] : A[k] - ((db + 1) * (db + 1) - s[k] * s[k]);
                     if (kDir > 0)
