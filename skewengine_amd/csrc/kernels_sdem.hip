@@ -393,16 +393,19 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
     }
     int ns = nck;
     for (int d0 = 0; ns > 0; d0 += 32) {
+        bool ends = false;  // some lane's row ends in this block (uniform)
 #pragma unroll
         for (int k = 0; k < kG; ++k) {
             if (k < ns) {
                 uint32_t w = scan_block<kDir, false>(row + jj[k], h[k], rtab, d0, 0, M[k]);
                 const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
                 // steps past the row end (the last e of the block) are recorded as
-                // visible; skipped when no lane's row ends in this block
-                if (__builtin_amdgcn_ballot_w64(myLen < d0 + 32) != 0) {
+                // visible; skipped when no lane's row ends in this block (a row
+                // ending at the block's last step adds no steps: e = 0)
+                if (__builtin_amdgcn_ballot_w64(myLen <= d0 + 32) != 0) {
                     const int e = min(max(d0 + 32 - myLen, 0), 31);
                     w |= (1u << e) - 1u;
+                    ends = true;
                 }
                 flush_block(w, d0, A[k], s[k]);
             }
@@ -415,6 +418,7 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
         // as it is, so its cv does not change; the bound costs about a
         // seventh of a block's instructions.
         const bool tb = ((d0 >> 5) & 1) == 0;
+        if (!tb && !ends) continue;  // no lane can retire: slots and masks stay as they are
         int T = 0;
 #pragma unroll
         for (int k = 0; k < kG; ++k) {
