@@ -39,7 +39,7 @@ OPS_PER_STEP = 4  # sub, divide, compare, max per ray step (src/kernelSDEM.h:191
 # (6 per step in the scan, then per 32-step block of a slot the popcount
 # flush, the end-of-row mask, the early-exit bound and the retire/compaction
 # bookkeeping, plus row staging and chunk tables).
-VALU_PER_LANE_STEP = 8.39
+VALU_PER_LANE_STEP = 8.26
 HBM_PEAK_GBS = 8000.0
 
 
