@@ -156,7 +156,9 @@ struct MidK {
 // Waves of a forward workgroup: kFWaves row bands; for C = 32 each band's two
 // 16-channel output blocks go to two waves, which halves the weights and
 // accumulators a wave holds (twice as many waves per SIMD hide the operand
-// reads' latency, for twice the operand reads).
+// reads' latency, for twice the operand reads).  Wave w takes band w / kSplit:
+// a band's waves are consecutive, so they run on different SIMDs and the
+// bands with a leftover group more spread over the SIMDs.
 template <int C>
 struct FwdSplit {
     static constexpr int kSplit = C == 32 ? 2 : 1;         // waves per band
@@ -493,8 +495,8 @@ __device__ __forceinline__ void cnn_tile(const float* __restrict__ r0, const flo
     const float* img = r0 + (size_t)b * n * n;
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar: wave-uniform tests branch
-    const int wave = F::kSplit > 1 ? wid % kFWaves : wid;     // row band
-    const int h0 = F::kSplit > 1 ? (wid / kFWaves) * H : 0;  // first channel block
+    const int wave = F::kSplit > 1 ? wid / F::kSplit : wid;     // row band
+    const int h0 = F::kSplit > 1 ? (wid % F::kSplit) * H : 0;  // first channel block
     const int j16 = lane & 15, kb = lane >> 4;  // A row / B column, and k block
     // prologue loads in flight together: W2 operands, W1, biases, W4, r0 tile
     bf16x8 w[H][K::kBlocks];
