@@ -198,6 +198,25 @@ __device__ __forceinline__ uint32_t scan_block(const float* rowj, float h, const
     return w;
 }
 
+// 16 steps d0 + u0 + 1 .. d0 + u0 + 16 of a block (no tail handling),
+// appended to w: the groups' slots run a block as two halves, so only 16
+// reciprocal pairs are live in scalar registers at a time.
+template <int kDir>
+__device__ __forceinline__ uint32_t scan_half(const float* rowj, float h, const float2* __restrict__ ytab, int d0,
+                                              int u0, float& M, uint32_t w) {
+#pragma unroll
+    for (int u = 1; u <= 16; ++u) {
+        const int d = d0 + u0 + u;
+        const float2 y = ytab[d];
+        const float a = rowj[kDir * d] - h;
+        const float q = __builtin_fmaf(a, y.x, a * y.y);
+        const float t = M - q;
+        M = fmaxf(M, q);
+        w = __builtin_amdgcn_alignbit(w, __float_as_uint(t), 31);
+    }
+    return w;
+}
+
 // Early exit.  After the steps up to db, no later target of a lane can be
 // visible once an upper bound of its quotient is <= M.  The remaining
 // targets (distances e = db+1 .. myLen) are split into four segments that
@@ -394,10 +413,17 @@ __device__ __forceinline__ void scan_group(const float* row, int L, float povh, 
     int ns = nck;
     for (int d0 = 0; ns > 0; d0 += 32) {
         bool ends = false;  // some lane's row ends in this block (uniform)
+        uint32_t wk[kG];
+#pragma unroll
+        for (int k = 0; k < kG; ++k)
+            if (k < ns) wk[k] = scan_half<kDir>(row + jj[k], h[k], ytab, d0, 0, M[k], 0u);
+#pragma unroll
+        for (int k = 0; k < kG; ++k)
+            if (k < ns) wk[k] = scan_half<kDir>(row + jj[k], h[k], ytab, d0, 16, M[k], wk[k]);
 #pragma unroll
         for (int k = 0; k < kG; ++k) {
             if (k < ns) {
-                uint32_t w = scan_block<kDir, false>(row + jj[k], h[k], rtab, d0, 0, M[k]);
+                uint32_t w = wk[k];
                 const int myLen = kDir > 0 ? L - 1 - jj[k] : jj[k];
                 // steps past the row end (the last e of the block) are recorded as
                 // visible; skipped when no lane's row ends in this block (a row
