@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_deskew_parts(const SecDesc* __restrict_
 // them for kRadonU sectors (rows past H as empty rows), then every thread
 // reads its 17 taps per sector from LDS: the values so_cell gives, added in
 // ascending sector order, so the result is the k_deskew one bit for bit.
-constexpr int kRadonU = 4, kRadonStage = 288;
+constexpr int kRadonU = 8, kRadonStage = 288;  // 18 KB of LDS: eight sectors per pass
 __device__ __forceinline__ void deskew_radon_body(const SecDesc* __restrict__ secs, const int* __restrict__ target,
                                                   const float* __restrict__ weight,
                                                   const uint32_t* __restrict__ fl, const float* __restrict__ q,
