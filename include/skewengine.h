@@ -233,9 +233,10 @@ int ske_batch_profile_get(ske_batch *b, const char *name, double *total_ms, int 
 int ske_batch_counter_get(ske_batch *b, const char *name, double *value);
 int ske_batch_profile_reset(ske_batch *b);
 /* Autotuner of the viewshed launch shape: times candidate (waves per
- * workgroup, persistent workgroups per CU, occupancy hint) triples on a
- * sample of the batch (one grid per distinct side, every 12th sector), keeps
- * the fastest and returns it. */
+ * workgroup, persistent workgroups per CU, occupancy hint, chunk group)
+ * shapes on a sample of the batch (one grid per distinct side, every 12th
+ * sector; a warm-up, then the best of three timed launches each), keeps the
+ * first candidate within 2% of the fastest and returns it. */
 int ske_batch_autotune(ske_batch *b, int kernel_id, float povh, int *waves, int *wgs_per_cu, int *occupancy,
                        double *best_ms);
 
